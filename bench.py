@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the smectic/isotropic quasi-incompressible time step on one MI355X.
+
+Workload (metric: grid-point updates/s at 512^3, mixed vs FP64 on one GPU):
+  tilted512 -- 512^3 grid, smectic slab tilted 15 degrees about x with an
+  undulated interface, seeded noise 1e-4; quasi step (24 r2r transforms per
+  step).  The mixed-precision path is the headline `value`; the FP64 path runs
+  beside it on the same input, and psi's relative L2 difference between the
+  two after the timed steps is reported.
+
+A step is one pass of the hot path (quasi.cpp:1017-1965) over the whole grid.
+Inputs are resident in HBM before timing starts; timing is wall clock around
+exactly `--steps` steps, with a device synchronisation on both sides.
+
+Also reported:
+  roofline      the pass kernel with the largest share of device time in the
+                timed region: algorithmic HBM bytes of its launches / their
+                HIP-event time (events recorded on the stream the passes run on)
+  cpu_baseline  the CPU oracle (oracle/, FP64 C restatement of the same step)
+                on this host's cores, on a bounded 64^3 sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "smaiso-quasi_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per grid-point update of the quasi step (24 transforms,
+# every pointwise stage fused into a transform pass): 194 field sweeps
+SWEEPS_PER_STEP = 194
+
+PASS_NAMES = {1: "inverse x pass", 2: "inverse y pass", 3: "inverse z pass",
+              4: "forward x pass", 5: "forward y pass", 6: "forward z pass"}
+PASS_SYMBOLS = {1: "k_pass<T, N, 0, true>", 2: "k_pass<T, N, 1, true>",
+                3: "k_pass<T, N, 2, true>", 4: "k_pass<T, N, 0, false>",
+                5: "k_pass<T, N, 1, false>", 6: "k_pass<T, N, 2, false>"}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_sim(qsm, ics, name, prec, shape):
+    d = ics.params_dict(name)
+    d["nx"], d["ny"], d["nz"] = shape
+    p = qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_})
+    return qsm.Sim(p, prec)
+
+
+def time_steps(sim, steps):
+    sim.step(0)  # synchronises the stream
+    t0 = time.perf_counter()
+    sim.step(steps)  # returns after a stream synchronisation
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(target_s=15.0):
+    """FP64 CPU oracle on a bounded sample: 64^3 interface workload, P = 4 halo map."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import ics
+    oracle.build()
+    d = ics.params_dict("interface64")
+    st = oracle.State(oracle.make_params(**d))
+    st.set_psi(ics.workload_psi("interface64"))
+    st.step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < target_s:
+        st.step(5)
+        steps += 5
+    dt = time.perf_counter() - t0
+    pts = d["nx"] * d["ny"] * d["nz"]
+    return {"value": pts * steps / dt, "unit": "grid-pt updates/s", "cores": oracle.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"oracle FP64 quasi step, 64^3 interface workload, {steps} steps in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512, help="cubic grid edge (default 512)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path is single-GPU by design)", file=sys.stderr)
+        return 2
+
+    import qsm
+    import ics
+    shape = (args.size,) * 3
+    name = "tilted512"
+    log(f"building initial condition {shape}")
+    psi0 = ics.workload_psi(name, shape=shape)
+
+    res = {}
+    fields = {}
+    for prec, tag in ((qsm.MIXED, "mixed"), (qsm.FP64, "fp64")):
+        if prec == qsm.FP64 and args.no_fp64:
+            continue
+        log(f"{tag}: create + setup")
+        sim = make_sim(qsm, ics, name, prec, shape)
+        sim.set_psi(psi0)
+        sim.step(args.warmup)
+        sim.timing(True)
+        log(f"{tag}: timing {args.steps} steps")
+        dt = time_steps(sim, args.steps)
+        kt = {q: sim.timing_get(q) for q in range(1, 7)}
+        sim.timing(False)
+        fields[tag] = sim.field("PSI")
+        diag = sim.diag()
+        sim.close()
+        res[tag] = {"s": dt, "kt": kt, "diag": diag}
+        log(f"{tag}: {dt / args.steps * 1e3:.2f} ms/step")
+
+    pts = float(np.prod(shape))
+    m = res["mixed"]
+    value = pts * args.steps / m["s"]
+    dom = max(m["kt"], key=lambda q: m["kt"][q]["ms"])
+    k = m["kt"][dom]
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+    out = {
+        "metric": "grid-pt updates/s at 512^3, mixed vs FP64 on 1 GPU; % HBM roofline; psi rel-L2",
+        "value": value,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["s"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (tilted-slab initial condition, seeded noise)",
+        "config": {"workload": f"{name}: quasi step, {shape[0]}x{shape[1]}x{shape[2]}, mixed precision "
+                               "(FP32 fields/transforms, FP64 psihat, coefficients, mean modes)",
+                   "grid": list(shape), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"{PASS_NAMES[dom]} ({PASS_SYMBOLS[dom]}, T=float, N={shape[0]})",
+                     "avg_launch_ms": k["ms"] / max(1, k["launches"]),
+                     "launches": k["launches"]},
+        "step_roofline": {"bytes_per_update": SWEEPS_PER_STEP * 4,
+                          "achieved_GBs": value * SWEEPS_PER_STEP * 4 / 1e9,
+                          "frac": value * SWEEPS_PER_STEP * 4 / 1e9 / HBM_PEAK_GBS},
+    }
+    if "fp64" in res:
+        f = res["fp64"]
+        v64 = pts * args.steps / f["s"]
+        a, b = fields["mixed"], fields["fp64"]
+        out["fp64"] = {"value": v64, "ms_per_step": f["s"] / args.steps * 1e3,
+                       "step_roofline_frac": v64 * SWEEPS_PER_STEP * 8 / 1e9 / HBM_PEAK_GBS}
+        out["mixed_over_fp64"] = value / v64
+        out["psi_rel_l2_mixed_vs_fp64"] = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
+        out["steps_compared"] = args.warmup + args.steps
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,168 @@
+/*
+ * qsm.h -- C ABI of the MI355X-native smectic-A / isotropic quasi-incompressible
+ * time step (library libqsm.so).
+ *
+ * The reference (evitral/smaiso-quasi) is one C++ main() per program that calls
+ * FFTW-MPI directly; its hot path is the while-loop body of quasi/quasi.cpp
+ * (and the same kernels in quasi/quasi-stb.cpp).  This header replaces two
+ * boundaries of that program:
+ *
+ *   1. the FFTW r2r boundary (transform level):
+ *        fftw_mpi_plan_r2r_3d  quasi/quasi.cpp:449-495   -> qsm_r2r3d_plan
+ *        fftw_execute          quasi/quasi.cpp:908...1964 -> qsm_r2r3d_execute
+ *        fftw_destroy_plan     quasi/quasi.cpp:2310-2319 -> qsm_r2r3d_destroy
+ *      with FFTW's unnormalised REDFT10/REDFT01/RODFT10/RODFT01 definitions
+ *      and FFTW's numeric kind values, so existing call sites keep their
+ *      constants;
+ *   2. the step loop of main() (step level): parameters (quasi.cpp:136-219),
+ *      setup (quasi.cpp:875-951), the step (quasi.cpp:1014-1965 /
+ *      quasi-stb.cpp:983-1968), diagnostics (quasi.cpp:1969-2031) and the
+ *      fields main() writes out.
+ *
+ * Conventions
+ *   - Fields are row-major [nx][ny][nz] (z fastest), the reference layout with
+ *     the rank slabs concatenated in rank order.  Host arrays are caller-owned
+ *     FP64 and copied in/out; device memory is owned by the library.
+ *   - Every function returns 0 (QSM_OK) or a negative QSM_ERR_* code; the
+ *     message of the last failure on the calling thread is qsm_last_error().
+ *   - A handle or plan is bound to one HIP stream; calls on one handle must
+ *     not overlap; distinct handles may be used from distinct threads.
+ *   - One GPU per process: the current HIP device at create time is used.
+ */
+#ifndef QSM_H
+#define QSM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define QSM_OK 0
+#define QSM_ERR_ARG (-1)          /* invalid argument */
+#define QSM_ERR_UNSUPPORTED (-2)  /* size or option without a compiled kernel */
+#define QSM_ERR_HIP (-3)          /* HIP runtime failure (message names it) */
+#define QSM_ERR_NONFINITE (-4)    /* a diagnostic became NaN/Inf (blow-up) */
+#define QSM_ERR_STATE (-5)        /* call out of order (e.g. step before set_psi) */
+#define QSM_ERR_NOMEM (-6)        /* device allocation failed */
+
+/* r2r kinds: the values of FFTW's fftw_r2r_kind (fftw3.h) */
+#define QSM_REDFT01 4 /* DCT-III: Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k+1/2)/N)     */
+#define QSM_REDFT10 5 /* DCT-II:  Y_k = 2 sum_j X_j cos(pi (j+1/2) k/N)                */
+#define QSM_RODFT01 8 /* DST-III: Y_k = (-1)^k X_{N-1} + 2 sum_{j<N-1} X_j sin(pi (j+1)(k+1/2)/N) */
+#define QSM_RODFT10 9 /* DST-II:  Y_k = 2 sum_j X_j sin(pi (j+1/2)(k+1)/N)             */
+
+/* arithmetic precision */
+#define QSM_FP64 0  /* every field and transform in FP64 */
+#define QSM_MIXED 1 /* FP32 fields, transforms and nonlinear terms; FP64 psihat
+                       state, coefficients, (0,0,0) modes and reductions */
+
+/* model variant */
+#define QSM_QUASI 0 /* quasi/quasi.cpp      */
+#define QSM_STB 1   /* quasi/quasi-stb.cpp  */
+
+/* Parameters: every constant the reference step reads.  Names follow
+ * quasi/quasi.cpp:159-219 (quasi-stb.cpp:149-220). */
+typedef struct {
+  int nx, ny, nz;      /* grid (quasi.cpp:164: 256^3)                              */
+  int variant;         /* QSM_QUASI or QSM_STB                                      */
+  int compat_ranks;    /* 0: exact stencils.  P > 0: reproduce the halo slab that an
+                          `mpirun -np P` reference run reads for S_x in the v_y/v_z
+                          projection (quasi.cpp:1639-1650); needs nx % P == 0        */
+  int nw;              /* points per wavelength: L = N 2 pi/(q0 nw) (quasi.cpp:202) */
+  int divv_switch;     /* quasi.cpp:161 (200): D enters after this step              */
+  int hydro;           /* stb advection switch (quasi-stb.cpp:155, 1)               */
+  double q0, alpha, beta, gamma, eps, nu, kp, rho0, amp, rho_m, dt;
+  double filter;       /* density filter exponent 1.57*1.57*8 (quasi.cpp:1261)      */
+  double growth_q;     /* stb perturbation wavenumber Qi (quasi-stb.cpp:157)        */
+} qsm_params;
+
+typedef struct {
+  double L1;        /* sum|psi-psi_old| / (dt sum|psi|)          quasi.cpp:2010 */
+  double mass;      /* sum rho dx^3                               quasi.cpp:2016 */
+  double energy;    /* sum rho(-eps psi^2/2 + alpha F^2/2 - beta psi^4/4 + gamma psi^6/6) dx^3 */
+  double interface_z; /* Lz (<rho> - rho0)/(rho_s - rho0): smectic fraction x Lz */
+  double growth;    /* stb: growth rate of mode (i*,0,k*) of the last step (stb:2147-2158) */
+  long long step;   /* steps taken since qsm_set_psi */
+} qsm_diag;
+
+/* fields of qsm_get_field (FP64 host copies; spectral ones in FFTW layout) */
+#define QSM_PSI 0
+#define QSM_RHO 1
+#define QSM_VX 2
+#define QSM_VY 3
+#define QSM_VZ 4
+#define QSM_P 5
+#define QSM_DIVV 6
+#define QSM_PSIHAT 7
+#define QSM_F 8
+#define QSM_G 9
+#define QSM_GX 10
+#define QSM_GY 11
+#define QSM_GZ 12
+#define QSM_RX 13
+#define QSM_RY 14
+#define QSM_RZ 15
+#define QSM_LX 16
+#define QSM_LY 17
+#define QSM_LZ 18
+#define QSM_RHOHAT 19
+#define QSM_NHAT 20
+#define QSM_PSIHAT_OLD 21
+#define QSM_PSI_OLD 22
+#define QSM_SX 23
+#define QSM_SY 24
+#define QSM_SZ 25
+#define QSM_DHAT 26
+
+typedef struct qsm_plan_s* qsm_plan;
+typedef struct qsm_handle_s* qsm_handle;
+
+const char* qsm_version(void);
+const char* qsm_last_error(void);
+
+/* Reference defaults of one variant (quasi.cpp:159-219 with argv = {nu=1,
+ * eps*1000=0...}); the caller overrides what the run needs. */
+int qsm_default_params(int variant, qsm_params* p);
+
+/* ---- transform level: replaces fftw_mpi_plan_r2r_3d / fftw_execute /
+ * fftw_destroy_plan (quasi.cpp:449-495, 908-1964, 2310-2319).  One kind per
+ * axis, unnormalised.  prec: QSM_FP64 (double arrays) or QSM_MIXED (float
+ * arrays).  flags: reserved, pass 0.  in == out is allowed. */
+int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, int prec, unsigned flags,
+                   qsm_plan* out);
+int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void* hip_stream);
+/* same with host arrays (copies in and out; the plan's precision decides
+ * the element type) */
+int qsm_r2r3d_execute_host(qsm_plan p, const void* h_in, void* h_out);
+int qsm_r2r3d_destroy(qsm_plan p);
+
+/* ---- step level: replaces the body of main() around the time loop */
+int qsm_create(const qsm_params* p, int prec, qsm_handle* out);
+/* setup of quasi.cpp:905-951 from psi(x) (FP64, [nx][ny][nz]) */
+int qsm_set_psi(qsm_handle h, const double* psi);
+/* advance nsteps steps (quasi.cpp:1017-1965 per step) */
+int qsm_step(qsm_handle h, int nsteps);
+/* run until L1 <= lim_l1 (checked every step_l1 steps, quasi.cpp:1014, 1972,
+ * 2010) or max_steps; *steps_done receives the steps taken */
+int qsm_run_l1(qsm_handle h, int max_steps, int step_l1, double lim_l1, int* steps_done);
+int qsm_get_field(qsm_handle h, int which, double* host);
+/* diagnostics of the current state (compensated FP64 sums over the fields
+ * of the last step); QSM_ERR_NONFINITE if any is NaN/Inf */
+int qsm_get_diag(qsm_handle h, qsm_diag* d);
+int qsm_destroy(qsm_handle h);
+
+/* ---- measurement support (no counterpart in the reference, which only
+ * prints total wall time at quasi.cpp:2292-2305): device time of the pass
+ * kernels, measured with HIP events recorded around every launch on the
+ * handle's stream while enabled.  which: 0 = every pass; 1/2/3 = the
+ * inverse (DCT-III/DST-III) x/y/z passes; 4/5/6 = the forward x/y/z passes.
+ * alg_bytes: algorithmic HBM bytes of those launches (fields read + fields
+ * written, at their element size).  qsm_timing_enable resets the counters. */
+int qsm_timing_enable(qsm_handle h, int enable);
+int qsm_timing_get(qsm_handle h, int which, double* total_ms, long long* launches,
+                   double* alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSM_H */

@@ -1,0 +1,785 @@
+// qsm_api.hip -- host side of libqsm: the C ABI of include/qsm.h.
+//
+// Owns device memory, wavenumber and twiddle tables, and sequences the pass
+// kernels of one time step (quasi/quasi.cpp:1017-1965, quasi-stb.cpp:986-1968).
+// Every reference loop between two fftw_execute calls is fused into the
+// first or last pass of the adjacent transform (qsm_ops.hpp), so a step is
+// 3 passes per 3D transform and nothing else (plus FP64 sums in mixed mode).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/qsm.h"
+#include "qsm_kernels.h"
+
+using namespace qsm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(QSM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+// ---- twiddle tables: e^{-2 pi i t/(4N)}, t < 4N, per (device, N, precision)
+std::mutex g_tw_mu;
+std::map<std::tuple<int, int, int>, void*> g_tw;
+
+const void* twiddles(int n, int fp32) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_tuple(dev, n, fp32);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) return it->second;
+  const int m = 4 * n;
+  std::vector<double> re(m), im(m);
+  for (int t = 0; t < m; ++t) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * t / m;
+    re[t] = (double)cosl(a);
+    im[t] = (double)sinl(a);
+  }
+  void* d = nullptr;
+  const size_t el = fp32 ? sizeof(float) : sizeof(double);
+  if (hipMalloc(&d, 2 * el * m) != hipSuccess) return nullptr;
+  if (fp32) {
+    std::vector<float> h(2 * m);
+    for (int t = 0; t < m; ++t) { h[2 * t] = (float)re[t]; h[2 * t + 1] = (float)im[t]; }
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  } else {
+    std::vector<double> h(2 * m);
+    for (int t = 0; t < m; ++t) { h[2 * t] = re[t]; h[2 * t + 1] = im[t]; }
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  }
+  g_tw[key] = d;
+  return d;
+}
+
+bool is_kind(int k) {
+  return k == QSM_REDFT01 || k == QSM_REDFT10 || k == QSM_RODFT01 || k == QSM_RODFT10;
+}
+bool kind_inv(int k) { return k == QSM_REDFT01 || k == QSM_RODFT01; }
+bool kind_dst(int k) { return k == QSM_RODFT01 || k == QSM_RODFT10; }
+
+// ---- per-launch timing with HIP events (pass categories: axis x direction)
+struct Timer {
+  bool on = false;
+  struct Rec { hipEvent_t a, b; int cat; double bytes; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double ms[8] = {0};
+  double bytes[8] = {0};
+  long long count[8] = {0};
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+  }
+  void drain() {
+    for (auto& r : recs) {
+      float t = 0;
+      hipEventSynchronize(r.b);
+      hipEventElapsedTime(&t, r.a, r.b);
+      ms[r.cat] += t; bytes[r.cat] += r.bytes; count[r.cat] += 1;
+      pool.push_back(r.a); pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+  void reset() {
+    drain();
+    for (int c = 0; c < 8; ++c) { ms[c] = 0; bytes[c] = 0; count[c] = 0; }
+  }
+  ~Timer() {
+    for (auto& r : recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+// ============================================================================
+// transform level
+
+struct qsm_plan_s {
+  int n[3];
+  int kind[3];
+  int fp32;
+  const void* tw[3];
+};
+
+extern "C" const char* qsm_version(void) { return "qsm 0.1 (gfx950)"; }
+extern "C" const char* qsm_last_error(void) { return g_err.c_str(); }
+
+extern "C" int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, int prec,
+                              unsigned flags, qsm_plan* out) {
+  if (!out) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: null output");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: bad size");
+  if (!is_kind(kx) || !is_kind(ky) || !is_kind(kz))
+    return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: unknown r2r kind");
+  if (prec != QSM_FP64 && prec != QSM_MIXED) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: bad precision");
+  if (flags != 0) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: flags must be 0");
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a)
+    if (!pass_supported(n[a]))
+      return fail(QSM_ERR_UNSUPPORTED, "qsm_r2r3d_plan: no pass kernel for length %d", n[a]);
+  auto* p = new qsm_plan_s;
+  const int k[3] = {kx, ky, kz};
+  p->fp32 = prec == QSM_MIXED;
+  for (int a = 0; a < 3; ++a) {
+    p->n[a] = n[a];
+    p->kind[a] = k[a];
+    p->tw[a] = twiddles(n[a], p->fp32);
+    if (!p->tw[a]) { delete p; return fail(QSM_ERR_NOMEM, "qsm_r2r3d_plan: twiddle table"); }
+  }
+  *out = p;
+  return QSM_OK;
+}
+
+extern "C" int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void* stream) {
+  if (!p || !d_in || !d_out) return fail(QSM_ERR_ARG, "qsm_r2r3d_execute: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  PassArgs A{};
+  A.geo = Geo{p->n[0], p->n[1], p->n[2], 0};
+  // x, y, z passes; the first reads d_in, the rest run in place on d_out
+  const int order[3] = {0, 1, 2};
+  for (int q = 0; q < 3; ++q) {
+    const int ax = order[q];
+    A.ld = LoadOp{};
+    A.ld.code = LD_PLAIN;
+    A.ld.a[0] = q == 0 ? d_in : d_out;
+    A.st = StoreOp{};
+    A.st.code = ST_PLAIN;
+    A.st.d[0] = d_out;
+    A.tw = p->tw[ax];
+    A.dst = kind_dst(p->kind[ax]);
+    hipError_t e = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), A, nullptr, s);
+    if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch failed: %s", hipGetErrorString(e));
+  }
+  return QSM_OK;
+}
+
+extern "C" int qsm_r2r3d_execute_host(qsm_plan p, const void* h_in, void* h_out) {
+  if (!p || !h_in || !h_out) return fail(QSM_ERR_ARG, "qsm_r2r3d_execute_host: null argument");
+  const size_t bytes = (size_t)p->n[0] * p->n[1] * p->n[2] * (p->fp32 ? 4 : 8);
+  void* d = nullptr;
+  HIPCHK(hipMalloc(&d, bytes));
+  int rc = QSM_OK;
+  if (hipMemcpy(d, h_in, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(QSM_ERR_HIP, "copy in");
+  if (rc == QSM_OK) rc = qsm_r2r3d_execute(p, d, d, nullptr);
+  if (rc == QSM_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(QSM_ERR_HIP, "execute");
+  if (rc == QSM_OK && hipMemcpy(h_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(QSM_ERR_HIP, "copy out");
+  hipFree(d);
+  return rc;
+}
+
+extern "C" int qsm_r2r3d_destroy(qsm_plan p) {
+  delete p;
+  return QSM_OK;
+}
+
+// ============================================================================
+// step level
+
+namespace {
+
+enum Fld {
+  F_NHAT_OLD, F_NTMP, F_RHOHAT, F_RHO_A, F_RHO_B, F_VX, F_VY, F_VZ, F_PSI_A, F_PSI_B,
+  F_F, F_G, F_GX, F_GY, F_GZ, F_RX, F_RY, F_RZ, F_LX, F_LY, F_LZ, F_DHAT, F_DIVV,
+  F_SX, F_SY, F_SZ, F_P, F_COUNT
+};
+
+// device FP64 scalars
+enum Dsc {
+  D_RHOSUM = 0,   // sum of rho* (mixed)
+  D_NSUM = 1,     // sum of N (mixed)
+  D_RHOHAT0 = 2,  // FP64 rhohat(0,0,0) (mixed)
+  D_NHAT0 = 3,    // FP64 N(0,0,0) of the last step (mixed)
+  D_DIAG = 8,     // 4 diagnostic sums
+  D_COUNT = 16
+};
+
+}  // namespace
+
+struct qsm_handle_s {
+  qsm_params p{};
+  int prec = 0, fp32 = 0;
+  size_t n = 0, esz = 8;
+  hipStream_t stream = nullptr;
+  Geo geo{};
+  Waves waves{};
+  Phys phys{};
+  double* wtab = nullptr;     // 6 wavenumber tables
+  const void* tw[3] = {nullptr, nullptr, nullptr};
+  void* f[F_COUNT] = {};
+  int psi_cur = F_PSI_A, psi_old = F_PSI_B, rho_cur = F_RHO_A, rho_old = F_RHO_B;
+  double* psihat = nullptr;   // FP64 state
+  double* psihat_old = nullptr;
+  double* part = nullptr;     // per-block partial sums
+  long part_n = 0;
+  double* dsc = nullptr;
+  long long step = 0;
+  bool ready = false;
+  double growth = 0;
+  Timer timer;
+};
+
+namespace {
+
+void fill_phys(qsm_handle_s* h) {
+  const qsm_params& p = h->p;
+  Phys& P = h->phys;
+  P.q02 = p.q0 * p.q0;
+  P.alpha = p.alpha; P.beta = p.beta; P.gamma = p.gamma; P.eps = p.eps; P.nu = p.nu;
+  P.kp = p.kp; P.rho0 = p.rho0; P.dt = p.dt; P.filter = p.filter; P.rho_m = p.rho_m;
+  P.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+  const double rs = p.kp * p.amp + p.rho0;
+  P.rho_s2 = rs * rs;
+  P.hydro = p.hydro;
+}
+
+// algorithmic HBM bytes of one pass: (fields read + fields written) x points,
+// FP64 state fields counted at 8 bytes
+double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool first_pass) {
+  const double n = (double)h->n, e = (double)h->esz;
+  double rd = 0, wr = 0;
+  if (!first_pass) rd = e;
+  else switch (L.code) {
+    case LD_PLAIN: rd = e; break;
+    case LD_PLAIN_D: case LD_COEF_F: case LD_COEF_G: case LD_SH: rd = 8; break;
+    case LD_ZERO: rd = 0; break;
+    case LD_GRAD: case LD_LAPGRAD: rd = L.flag ? 8 : e; break;
+    case LD_RHOSTAR: rd = 4 * e; break;
+    case LD_DIVV: rd = L.flag ? 8 * e : 0; break;
+    case LD_STRESS: rd = 5 * e; break;
+    case LD_PRESSURE: rd = 3 * e; break;
+    case LD_VEL: rd = (L.a[3] ? 4 : 3) * e; break;
+    case LD_SCALE: rd = e; break;
+    case LD_NONLIN: rd = (L.flag ? 14 : 10) * e; break;
+    case LD_STRESS_STB: rd = 3 * e; break;
+    case LD_NONLIN_STB: rd = (h->p.hydro == 1 ? 7 : 1) * e; break;
+    default: rd = e;
+  }
+  switch (S.code) {
+    case ST_PLAIN: case ST_FILTER: wr = e; break;
+    case ST_UPDATE: case ST_UPDATE_STB: rd += 16 + e; wr = 8 + e; break;
+    default: wr = e;
+  }
+  return (rd + wr) * n;
+}
+
+int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, const StoreOp& S,
+             bool first_pass, double* part) {
+  PassArgs A{};
+  A.geo = h->geo;
+  A.waves = h->waves;
+  A.phys = h->phys;
+  A.ld = L;
+  A.st = S;
+  A.tw = h->tw[axis];
+  A.part = part;
+  A.dst = dst ? 1 : 0;
+  const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
+  Timer& T = h->timer;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+  hipError_t e = launch_pass(h->fp32, n, axis, inv, A, nullptr, h->stream);
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch (axis %d, n %d): %s", axis, n, hipGetErrorString(e));
+  if (T.on) {
+    hipEventRecord(eb, h->stream);
+    T.recs.push_back({ea, eb, 1 + axis + (inv ? 0 : 3), op_bytes(h, L, S, first_pass)});
+  }
+  return QSM_OK;
+}
+
+LoadOp plain_load(const void* src) {
+  LoadOp L{};
+  L.code = LD_PLAIN;
+  L.a[0] = src;
+  return L;
+}
+StoreOp plain_store(void* dst) {
+  StoreOp S{};
+  S.code = ST_PLAIN;
+  S.d[0] = dst;
+  return S;
+}
+
+// Inverse 3D transform (DCT-III on every axis except DST-III along dst_axis):
+// x pass with the fused load operator L into `out`, then y and z in place.
+int inv3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* out) {
+  int rc = run_pass(h, 0, true, dst_axis == 0, L, plain_store(out), true, nullptr);
+  if (rc) return rc;
+  rc = run_pass(h, 1, true, dst_axis == 1, plain_load(out), plain_store(out), false, nullptr);
+  if (rc) return rc;
+  return run_pass(h, 2, true, dst_axis == 2, plain_load(out), plain_store(out), false, nullptr);
+}
+
+// Forward 3D transform (DCT-II, DST-II along dst_axis): z pass with the fused
+// load operator L into `tmp`, y in place, x pass from `tmp` with store operator S.
+// dc_slot >= 0 (mixed precision): FP64 sum of the loaded values into dsc[dc_slot].
+int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreOp& S, int dc_slot) {
+  double* part = dc_slot >= 0 ? h->part : nullptr;
+  int rc = run_pass(h, 2, false, dst_axis == 2, L, plain_store(tmp), true, part);
+  if (rc) return rc;
+  if (part) {
+    const long nb = pass_blocks(h->fp32, h->p.nz, 2, h->geo);
+    hipError_t e = launch_finish(part, nb, h->dsc + dc_slot, h->stream);
+    if (e != hipSuccess) return fail(QSM_ERR_HIP, "dc sum: %s", hipGetErrorString(e));
+  }
+  rc = run_pass(h, 1, false, dst_axis == 1, plain_load(tmp), plain_store(tmp), false, nullptr);
+  if (rc) return rc;
+  return run_pass(h, 0, false, dst_axis == 0, plain_load(tmp), S, false, nullptr);
+}
+
+void* F(qsm_handle_s* h, int f) { return h->f[f]; }
+
+int quasi_step(qsm_handle_s* h) {
+  const long long n = ++h->step;
+  const qsm_params& p = h->p;
+  const bool mixed = h->fp32;
+  int rc;
+  if (n >= 2) std::swap(h->rho_cur, h->rho_old);  // rho_old <- rho (quasi.cpp:1289)
+  void *rho = F(h, h->rho_cur), *rho_old = F(h, h->rho_old), *psi = F(h, h->psi_cur);
+
+  // S1: F, G (quasi.cpp:1034-1057)
+  LoadOp L{};
+  L.code = LD_COEF_F; L.a[0] = h->psihat;
+  if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
+  L.code = LD_COEF_G;
+  if ((rc = inv3(h, -1, L, F(h, F_G)))) return rc;
+
+  // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 0; L.a[0] = F(h, F_RHOHAT);
+    if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
+    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
+  }
+
+  // S3: density law + filter; rho (quasi.cpp:1237-1268)
+  L = LoadOp{}; L.code = LD_RHOSTAR;
+  L.a[0] = psi; L.a[1] = F(h, F_GX); L.a[2] = F(h, F_GY); L.a[3] = F(h, F_GZ);
+  StoreOp S{}; S.code = ST_FILTER; S.d[0] = F(h, F_RHOHAT);
+  if (mixed) { S.dc = h->dsc + D_RHOSUM; S.dc_out = h->dsc + D_RHOHAT0; }
+  if ((rc = fwd3(h, -1, L, F(h, F_RHOHAT), S, mixed ? D_RHOSUM : -1))) return rc;
+  if ((rc = inv3(h, -1, plain_load(F(h, F_RHOHAT)), rho))) return rc;
+
+  // S4: divergence (quasi.cpp:1273-1293)
+  L = LoadOp{}; L.code = LD_DIVV; L.flag = n > p.divv_switch;
+  L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
+  L.a[5] = F(h, F_RX); L.a[6] = F(h, F_RY); L.a[7] = F(h, F_RZ);
+  if ((rc = fwd3(h, -1, L, F(h, F_DHAT), plain_store(F(h, F_DHAT)), -1))) return rc;
+
+  // S5: stress (quasi.cpp:1298-1330)
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
+    L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F); L.a[3] = F(h, F_GX + c); L.a[4] = F(h, F_LX + c);
+    if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+  }
+
+  // S6: pressure (quasi.cpp:1336-1418)
+  L = LoadOp{}; L.code = LD_PRESSURE; L.flag = 0;
+  L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
+  if ((rc = inv3(h, -1, L, F(h, F_P)))) return rc;
+
+  // S7: velocity projection (quasi.cpp:1425-1884)
+  const int n0 = p.compat_ranks > 0 ? p.nx / p.compat_ranks : 0;
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_VEL; L.axis = c; L.n0 = n0;
+    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = F(h, F_DHAT);
+    if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+  }
+
+  // S8: divv output (quasi.cpp:1888-1900)
+  L = LoadOp{}; L.code = LD_SCALE; L.a[0] = F(h, F_DHAT);
+  if ((rc = inv3(h, -1, L, F(h, F_DIVV)))) return rc;
+
+  // S9 + S10: N, its transform, and the update fused into the last pass
+  L = LoadOp{}; L.code = LD_NONLIN; L.flag = n > p.divv_switch + 1;
+  L.a[0] = rho; L.a[1] = psi; L.a[2] = F(h, F_F); L.a[3] = F(h, F_G);
+  L.a[4] = F(h, F_VX); L.a[5] = F(h, F_VY); L.a[6] = F(h, F_VZ);
+  L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
+  L.a[10] = F(h, F_P); L.a[11] = F(h, F_RX); L.a[12] = F(h, F_RY); L.a[13] = F(h, F_RZ);
+  S = StoreOp{}; S.code = ST_UPDATE; S.first = n == 1;
+  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.s[0] = h->psihat;
+  if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
+  if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
+  std::swap(h->psihat, h->psihat_old);
+
+  // psi = IDCT(psihat) into the psi_old buffer, then swap (psi_old <- psi)
+  L = LoadOp{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
+  if ((rc = inv3(h, -1, L, F(h, h->psi_old)))) return rc;
+  std::swap(h->psi_cur, h->psi_old);
+  return QSM_OK;
+}
+
+int stb_step(qsm_handle_s* h) {
+  const long long n = ++h->step;
+  const bool mixed = h->fp32;
+  int rc;
+  void* psi = F(h, h->psi_cur);
+  // sh = IDCT(-aLin psihat) (quasi-stb.cpp:1015-1038), kept in F
+  LoadOp L{};
+  L.code = LD_SH; L.a[0] = h->psihat;
+  if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
+  }
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
+    L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);
+    if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+  }
+  const int n0 = h->p.compat_ranks > 0 ? h->p.nx / h->p.compat_ranks : 0;
+  for (int c = 0; c < 3; ++c) {
+    L = LoadOp{}; L.code = LD_VEL; L.axis = c; L.n0 = n0;
+    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = nullptr;
+    if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+  }
+  L = LoadOp{}; L.code = LD_NONLIN_STB;
+  L.a[0] = psi; L.a[1] = F(h, F_VX); L.a[2] = F(h, F_VY); L.a[3] = F(h, F_VZ);
+  L.a[4] = F(h, F_GX); L.a[5] = F(h, F_GY); L.a[6] = F(h, F_GZ);
+  StoreOp S{}; S.code = ST_UPDATE_STB; S.first = n == 1;
+  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.s[0] = h->psihat;
+  if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
+  if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
+  std::swap(h->psihat, h->psihat_old);
+  L = LoadOp{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
+  if ((rc = inv3(h, -1, L, F(h, h->psi_old)))) return rc;
+  std::swap(h->psi_cur, h->psi_old);
+  return QSM_OK;
+}
+
+size_t idx3(const qsm_params& p, int i, int j, int k) {
+  return ((size_t)i * p.ny + j) * p.nz + k;
+}
+
+int stb_growth(qsm_handle_s* h) {
+  // quasi-stb.cpp:2147-2158 at mode (i*, 0, k*); i* from q_x[1], k* from q_z[1]
+  const qsm_params& p = h->p;
+  if (h->step < 2) { h->growth = 0; return QSM_OK; }
+  const double Lx = p.nx * 2.0 * M_PI / (p.q0 * p.nw), Lz = p.nz * 2.0 * M_PI / (p.q0 * p.nw);
+  const double qx1 = M_PI * 1 / ((Lx / p.nx) * p.nx), qz1 = M_PI * 1 / ((Lz / p.nz) * p.nz);
+  const long is = lround(p.growth_q / qx1), ks = lround(p.q0 / qz1);
+  if (is >= p.nx || ks >= p.nz) { h->growth = 0; return QSM_OK; }
+  const size_t id = idx3(p, (int)is, 0, (int)ks);
+  double a = 0, b = 0;
+  HIPCHK(hipMemcpyAsync(&a, h->psihat + id, 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&b, h->psihat_old + id, 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->growth = (a - b) / (p.dt * b);
+  return QSM_OK;
+}
+
+void free_handle(qsm_handle_s* h) {
+  if (!h) return;
+  for (int f = 0; f < F_COUNT; ++f) if (h->f[f]) hipFree(h->f[f]);
+  if (h->psihat) hipFree(h->psihat);
+  if (h->psihat_old) hipFree(h->psihat_old);
+  if (h->part) hipFree(h->part);
+  if (h->dsc) hipFree(h->dsc);
+  if (h->wtab) hipFree(h->wtab);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // namespace
+
+extern "C" int qsm_default_params(int variant, qsm_params* p) {
+  if (!p) return fail(QSM_ERR_ARG, "null params");
+  std::memset(p, 0, sizeof *p);
+  p->variant = variant;
+  p->nw = 8; p->q0 = 1.0; p->alpha = 1.0; p->hydro = 1;
+  p->filter = 1.57 * 1.57 * 8;
+  p->compat_ranks = 0;
+  if (variant == QSM_QUASI) {
+    // quasi.cpp:159-219
+    p->nx = p->ny = p->nz = 256;
+    p->divv_switch = 200;
+    p->gamma = 1.0; p->beta = 2.0; p->eps = 0.0; p->nu = 1.0;
+    p->amp = 1.34164; p->rho0 = 0.005; p->kp = 0.5;
+    p->rho_m = (p->kp * p->amp + p->rho0) / 2;
+    p->dt = 0.001;
+  } else if (variant == QSM_STB) {
+    // quasi-stb.cpp:149-220
+    p->nx = p->ny = p->nz = 512;
+    p->divv_switch = 200;
+    p->gamma = 3.0; p->beta = 0.4; p->eps = 0.0; p->nu = 0.1;
+    p->amp = 0.3464101615; p->rho0 = 0.1; p->kp = 0.5; p->rho_m = 0.5;
+    p->dt = 0.0005;
+    p->growth_q = 0.5;
+  } else {
+    return fail(QSM_ERR_ARG, "unknown variant %d", variant);
+  }
+  return QSM_OK;
+}
+
+extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
+  if (!pp || !out) return fail(QSM_ERR_ARG, "qsm_create: null argument");
+  *out = nullptr;
+  const qsm_params& p = *pp;
+  if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(QSM_ERR_ARG, "qsm_create: grid too small");
+  if (p.variant != QSM_QUASI && p.variant != QSM_STB) return fail(QSM_ERR_ARG, "qsm_create: bad variant");
+  if (prec != QSM_FP64 && prec != QSM_MIXED) return fail(QSM_ERR_ARG, "qsm_create: bad precision");
+  if (p.compat_ranks < 0 || (p.compat_ranks > 0 && p.nx % p.compat_ranks != 0))
+    return fail(QSM_ERR_ARG, "qsm_create: compat_ranks must divide nx");
+  if (!(p.dt > 0) || !(p.nu > 0) || p.nw <= 0 || !(p.q0 > 0))
+    return fail(QSM_ERR_ARG, "qsm_create: dt, nu, q0 and nw must be positive");
+  const int nn[3] = {p.nx, p.ny, p.nz};
+  for (int a = 0; a < 3; ++a)
+    if (!pass_supported(nn[a])) return fail(QSM_ERR_UNSUPPORTED, "qsm_create: no pass kernel for length %d", nn[a]);
+
+  auto* h = new qsm_handle_s;
+  h->p = p;
+  h->prec = prec;
+  h->fp32 = prec == QSM_MIXED;
+  h->esz = h->fp32 ? 4 : 8;
+  h->n = (size_t)p.nx * p.ny * p.nz;
+  h->geo = Geo{p.nx, p.ny, p.nz, 0};
+  fill_phys(h);
+  auto bail = [&](int code, const char* what) {
+    free_handle(h);
+    return fail(code, "qsm_create: %s", what);
+  };
+  if (hipStreamCreate(&h->stream) != hipSuccess) return bail(QSM_ERR_HIP, "stream");
+  // wavenumbers, quasi.cpp:397-431 (same expressions: pi m/(dx N), pi (m+1)/(dx N))
+  std::vector<double> w;
+  for (int a = 0; a < 3; ++a) {
+    const int N = nn[a];
+    const double L = N * 2.0 * M_PI / (p.q0 * p.nw), d = L / N;
+    for (int m = 0; m < N; ++m) w.push_back(1.0 * M_PI * m / (d * N));
+    for (int m = 0; m < N; ++m) w.push_back(1.0 * M_PI * (m + 1) / (d * N));
+  }
+  if (hipMalloc(&h->wtab, w.size() * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "wavenumber tables");
+  if (hipMemcpy(h->wtab, w.data(), w.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(QSM_ERR_HIP, "wavenumber upload");
+  h->waves.qx = h->wtab; h->waves.sx = h->wtab + p.nx;
+  h->waves.qy = h->wtab + 2 * p.nx; h->waves.sy = h->waves.qy + p.ny;
+  h->waves.qz = h->wtab + 2 * p.nx + 2 * p.ny; h->waves.sz = h->waves.qz + p.nz;
+  for (int a = 0; a < 3; ++a) {
+    h->tw[a] = twiddles(nn[a], h->fp32);
+    if (!h->tw[a]) return bail(QSM_ERR_NOMEM, "twiddle table");
+  }
+  for (int f = 0; f < F_COUNT; ++f)
+    if (hipMalloc(&h->f[f], h->n * h->esz) != hipSuccess) return bail(QSM_ERR_NOMEM, "field");
+  if (hipMalloc(&h->psihat, h->n * 8) != hipSuccess || hipMalloc(&h->psihat_old, h->n * 8) != hipSuccess)
+    return bail(QSM_ERR_NOMEM, "psihat");
+  long nb = pass_blocks(h->fp32, p.nz, 2, h->geo);
+  h->part_n = nb > 4096 ? nb : 4096;
+  if (hipMalloc(&h->part, h->part_n * 4 * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "partials");
+  if (hipMalloc(&h->dsc, D_COUNT * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "scalars");
+  if (hipMemset(h->dsc, 0, D_COUNT * 8) != hipSuccess) return bail(QSM_ERR_HIP, "memset");
+  *out = h;
+  return QSM_OK;
+}
+
+extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
+  if (!h || !psi) return fail(QSM_ERR_ARG, "qsm_set_psi: null argument");
+  const size_t n = h->n;
+  for (int f = 0; f < F_COUNT; ++f) HIPCHK(hipMemsetAsync(h->f[f], 0, n * h->esz, h->stream));
+  HIPCHK(hipMemsetAsync(h->dsc, 0, D_COUNT * 8, h->stream));
+  h->psi_cur = F_PSI_A; h->psi_old = F_PSI_B; h->rho_cur = F_RHO_A; h->rho_old = F_RHO_B;
+  // psi (FP64 host) -> psihat = scale DCT(psi) in FP64, whatever the precision
+  HIPCHK(hipMemcpyAsync(h->psihat_old, psi, n * 8, hipMemcpyHostToDevice, h->stream));
+  // forward FP64 DCT of psi: z, y, x passes on psihat; scale in the last store
+  const int saved_fp32 = h->fp32;
+  h->fp32 = 0;
+  const void* tw_saved[3] = {h->tw[0], h->tw[1], h->tw[2]};
+  const int nn[3] = {h->p.nx, h->p.ny, h->p.nz};
+  for (int a = 0; a < 3; ++a) h->tw[a] = twiddles(nn[a], 0);
+  int rc = run_pass(h, 2, false, false, plain_load(h->psihat_old), plain_store(h->psihat), true, nullptr);
+  if (!rc) rc = run_pass(h, 1, false, false, plain_load(h->psihat), plain_store(h->psihat), false, nullptr);
+  // last pass: scale via the filter store with filter exponent 0 (exp(0) = 1)
+  StoreOp S{}; S.code = ST_FILTER; S.d[0] = h->psihat;
+  const double filt = h->phys.filter;
+  h->phys.filter = 0.0;
+  if (!rc) rc = run_pass(h, 0, false, false, plain_load(h->psihat), S, false, nullptr);
+  h->phys.filter = filt;
+  for (int a = 0; a < 3; ++a) h->tw[a] = tw_saved[a];
+  h->fp32 = saved_fp32;
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h->psihat_old, h->psihat, n * 8, hipMemcpyDeviceToDevice, h->stream));
+  // psi = IDCT(psihat) (quasi.cpp:924-927), psi_old = psi
+  LoadOp L{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
+  if ((rc = inv3(h, -1, L, F(h, h->psi_cur)))) return rc;
+  HIPCHK(hipMemcpyAsync(F(h, h->psi_old), F(h, h->psi_cur), n * h->esz, hipMemcpyDeviceToDevice, h->stream));
+  if (h->p.variant == QSM_STB) {
+    // rho = 1 in the pure smectic (quasi-stb.cpp:1232): kept for the mass diagnostic
+    if (h->fp32) {
+      std::vector<float> ones(n, 1.0f);
+      HIPCHK(hipMemcpy(F(h, h->rho_cur), ones.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> ones(n, 1.0);
+      HIPCHK(hipMemcpy(F(h, h->rho_cur), ones.data(), n * 8, hipMemcpyHostToDevice));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->step = 0;
+  h->growth = 0;
+  h->ready = true;
+  return QSM_OK;
+}
+
+extern "C" int qsm_step(qsm_handle h, int nsteps) {
+  if (!h) return fail(QSM_ERR_ARG, "qsm_step: null handle");
+  if (!h->ready) return fail(QSM_ERR_STATE, "qsm_step: call qsm_set_psi first");
+  if (nsteps < 0) return fail(QSM_ERR_ARG, "qsm_step: negative step count");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = h->p.variant == QSM_STB ? stb_step(h) : quasi_step(h);
+    if (rc) return rc;
+  }
+  if (h->p.variant == QSM_STB && nsteps > 0) {
+    int rc = stb_growth(h);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return QSM_OK;
+}
+
+extern "C" int qsm_get_diag(qsm_handle h, qsm_diag* d) {
+  if (!h || !d) return fail(QSM_ERR_ARG, "qsm_get_diag: null argument");
+  if (!h->ready) return fail(QSM_ERR_STATE, "qsm_get_diag: no state");
+  const qsm_params& p = h->p;
+  int nb = (int)std::min<long>(1024, (long)(h->n + 255) / 256);
+  if (nb > h->part_n) nb = (int)h->part_n;
+  hipError_t e = launch_diag(h->fp32, F(h, h->psi_cur), F(h, h->psi_old), F(h, h->rho_cur), F(h, F_F),
+                             (long)h->n, h->phys, h->part, nb, h->dsc + D_DIAG, h->stream);
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "diag launch: %s", hipGetErrorString(e));
+  double s[4];
+  HIPCHK(hipMemcpyAsync(s, h->dsc + D_DIAG, 32, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double dx = 2.0 * M_PI / (p.q0 * p.nw);
+  d->L1 = s[0] / (p.dt * s[1]);
+  d->mass = s[2] * dx * dx * dx;
+  d->energy = s[3] * dx * dx * dx;
+  const double vol = (double)p.nx * p.ny * p.nz * dx * dx * dx;
+  const double rs = p.kp * p.amp + p.rho0;
+  const double Lz = p.nz * dx;
+  d->interface_z = (rs != p.rho0) ? Lz * (d->mass / vol - p.rho0) / (rs - p.rho0) : 0.0;
+  d->growth = h->growth;
+  d->step = h->step;
+  if (!std::isfinite(d->L1) && h->step > 0) return fail(QSM_ERR_NONFINITE, "L1 is not finite at step %lld", h->step);
+  if (!std::isfinite(d->mass) || !std::isfinite(d->energy))
+    return fail(QSM_ERR_NONFINITE, "diagnostics not finite at step %lld", h->step);
+  return QSM_OK;
+}
+
+extern "C" int qsm_run_l1(qsm_handle h, int max_steps, int step_l1, double lim_l1, int* steps_done) {
+  if (!h || step_l1 <= 0 || max_steps < 0) return fail(QSM_ERR_ARG, "qsm_run_l1: bad argument");
+  int done = 0;
+  double L1 = 1.0;  // quasi.cpp:963
+  while (L1 > lim_l1 && done < max_steps) {
+    const int k = std::min(step_l1, max_steps - done);
+    int rc = qsm_step(h, k);
+    if (rc) { if (steps_done) *steps_done = done; return rc; }
+    done += k;
+    if (k == step_l1) {
+      qsm_diag d;
+      rc = qsm_get_diag(h, &d);
+      if (rc) { if (steps_done) *steps_done = done; return rc; }
+      L1 = d.L1;
+    }
+  }
+  if (steps_done) *steps_done = done;
+  return QSM_OK;
+}
+
+extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
+  if (!h || !host) return fail(QSM_ERR_ARG, "qsm_get_field: null argument");
+  const void* src = nullptr;
+  bool is_double = !h->fp32;
+  switch (which) {
+    case QSM_PSI: src = F(h, h->psi_cur); break;
+    case QSM_PSI_OLD: src = F(h, h->psi_old); break;
+    case QSM_RHO: src = F(h, h->rho_cur); break;
+    case QSM_VX: src = F(h, F_VX); break;
+    case QSM_VY: src = F(h, F_VY); break;
+    case QSM_VZ: src = F(h, F_VZ); break;
+    case QSM_P: src = F(h, F_P); break;
+    case QSM_DIVV: src = F(h, F_DIVV); break;
+    case QSM_PSIHAT: src = h->psihat; is_double = true; break;
+    case QSM_PSIHAT_OLD: src = h->psihat_old; is_double = true; break;
+    case QSM_F: src = F(h, F_F); break;
+    case QSM_G: src = F(h, F_G); break;
+    case QSM_GX: case QSM_GY: case QSM_GZ: src = F(h, F_GX + (which - QSM_GX)); break;
+    case QSM_RX: case QSM_RY: case QSM_RZ: src = F(h, F_RX + (which - QSM_RX)); break;
+    case QSM_LX: case QSM_LY: case QSM_LZ: src = F(h, F_LX + (which - QSM_LX)); break;
+    case QSM_RHOHAT: src = F(h, F_RHOHAT); break;
+    case QSM_NHAT: src = F(h, F_NHAT_OLD); break;
+    case QSM_SX: case QSM_SY: case QSM_SZ: src = F(h, F_SX + (which - QSM_SX)); break;
+    case QSM_DHAT: src = F(h, F_DHAT); break;
+    default: return fail(QSM_ERR_ARG, "qsm_get_field: unknown field %d", which);
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (is_double) {
+    HIPCHK(hipMemcpy(host, src, h->n * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(h->n);
+    HIPCHK(hipMemcpy(tmp.data(), src, h->n * 4, hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < h->n; ++t) host[t] = tmp[t];
+  }
+  // mixed precision keeps the (0,0,0) modes in FP64 beside the FP32 arrays
+  if (h->fp32 && (which == QSM_RHOHAT || which == QSM_NHAT)) {
+    double v;
+    HIPCHK(hipMemcpy(&v, h->dsc + (which == QSM_RHOHAT ? D_RHOHAT0 : D_NHAT0), 8, hipMemcpyDeviceToHost));
+    if (h->step > 0) host[0] = v;
+  }
+  return QSM_OK;
+}
+
+extern "C" int qsm_destroy(qsm_handle h) {
+  if (h) hipStreamSynchronize(h->stream);
+  free_handle(h);
+  return QSM_OK;
+}
+
+extern "C" int qsm_timing_enable(qsm_handle h, int enable) {
+  if (!h) return fail(QSM_ERR_ARG, "null handle");
+  h->timer.reset();
+  h->timer.on = enable != 0;
+  return QSM_OK;
+}
+
+extern "C" int qsm_timing_get(qsm_handle h, int which, double* total_ms, long long* launches,
+                              double* alg_bytes) {
+  if (!h || which < 0 || which > 6) return fail(QSM_ERR_ARG, "qsm_timing_get: bad argument");
+  h->timer.drain();
+  double ms = 0, by = 0;
+  long long c = 0;
+  for (int q = 1; q <= 6; ++q)
+    if (which == 0 || which == q) { ms += h->timer.ms[q]; by += h->timer.bytes[q]; c += h->timer.count[q]; }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = c;
+  if (alg_bytes) *alg_bytes = by;
+  return QSM_OK;
+}

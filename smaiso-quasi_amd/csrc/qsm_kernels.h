@@ -1,0 +1,40 @@
+// qsm_kernels.h -- host-visible interface of the device kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qsm_ops.hpp"
+
+// Pencil lengths with a compiled pass kernel.  Each is N = 2M with M a
+// product of the radices 2, 3, 4, 8, 16 (qsm_fft.hpp).
+#define QSM_FOR_EACH_N(X) \
+  X(8) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+
+namespace qsm {
+
+struct PassArgs {
+  Geo geo;
+  Waves waves;
+  Phys phys;
+  LoadOp ld;
+  StoreOp st;
+  const void* tw;   // e^{-2 pi i t/(4N)} table, t < 4N, for this axis length (field precision)
+  double* part;     // optional per-block FP64 sums of the loaded values
+  int dst;          // 1: RODFT10/RODFT01 along this axis, 0: REDFT10/REDFT01
+  int pad_;
+};
+
+// One pass of a 3D r2r transform along `axis` (0 = x, 1 = y, 2 = z) of length n.
+// inv = REDFT01/RODFT01 (DCT-III/DST-III); otherwise REDFT10/RODFT10.
+hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, const PassArgs& A, long* nblocks,
+                       hipStream_t s);
+bool pass_supported(int n);
+int pass_tile_k(int prec_fp32, int n, int axis);
+long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
+
+// FP64 compensated reductions
+hipError_t launch_diag(int prec_fp32, const void* psi, const void* pold, const void* rho,
+                       const void* F, long n, const Phys& P, double* part, int nblocks,
+                       double* out4, hipStream_t s);
+hipError_t launch_finish(const double* part, long nparts, double* out, hipStream_t s);
+
+}  // namespace qsm

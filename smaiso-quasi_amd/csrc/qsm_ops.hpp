@@ -1,0 +1,297 @@
+// qsm_ops.hpp -- argument blocks of one r2r pass and the pointwise operators
+// fused into the first and last passes of the step's transforms.
+//
+// Every pointwise loop of the reference step that sits between two
+// fftw_execute calls becomes either a LOAD operator (evaluated while the first
+// pass of the following transform reads its tile) or a STORE operator
+// (evaluated while the last pass of the preceding transform writes its tile).
+// Coefficients are evaluated in FP64 in registers from the 1-D wavenumber
+// tables; none of the reference's coefficient arrays is materialised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qsm {
+
+// geometry of the [nx][ny][nz] field (z fastest)
+struct Geo {
+  int nx, ny, nz;
+  int pad_;
+};
+
+// FP64 wavenumber tables, quasi.cpp:397-431: q[m] = pi m/L, s[m] = pi (m+1)/L
+struct Waves {
+  const double *qx, *sx, *qy, *sy, *qz, *sz;
+};
+
+// physical constants of the step (quasi.cpp:159-219, quasi-stb.cpp:149-220)
+struct Phys {
+  double q02, alpha, beta, gamma, eps, nu, kp, rho0, dt, scale, filter, rho_s2, rho_m;
+  int hydro;
+  int pad_;
+};
+
+enum LoadCode : int {
+  LD_PLAIN = 0,     // a0[idx] (field type T)
+  LD_PLAIN_D,       // (T) a0[idx], a0 is double (psihat)
+  LD_ZERO,          // 0
+  LD_COEF_F,        // alpha (q0^2 - K^2) psihat                          quasi.cpp:1042
+  LD_COEF_G,        // -K^2 alpha (q0^2 - K^2) psihat                     quasi.cpp:1044-1045
+  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1          quasi.cpp:1124-1171
+  LD_LAPGRAD,       // -K_c^2 (-s_c[m] src[m+1])                          quasi.cpp:1176-1191
+  LD_SH,            // -aLin psihat (stb)                                 quasi-stb.cpp:1023
+  LD_RHOSTAR,       // kp sqrt(q0^2 psi^2 + |g|^2) + rho0                 quasi.cpp:1243-1247
+  LD_DIVV,          // -((rho-rho_old)/dt + v.r)/(0.5(rho+rho_old))       quasi.cpp:1280-1285
+  LD_STRESS,        // (rho G) g_c - (rho F) l_c                          quasi.cpp:1305-1317
+  LD_PRESSURE,      // scale (-(q.S_shifted)/K^2)                         quasi.cpp:1385-1415
+  LD_VEL,           // Stokes/Leray projection of S                       quasi.cpp:1522-1872
+  LD_SCALE,         // scale a0[idx]                                      quasi.cpp:1891-1896
+  LD_NONLIN,        // N of quasi.cpp:1914-1928
+  LD_STRESS_STB,    // rho mu g_c, mu = sh - beta psi^3 + gamma psi^5     quasi-stb.cpp:1295-1307
+  LD_NONLIN_STB,    // beta psi^3 - gamma psi^5 - v.g                     quasi-stb.cpp:1903-1914
+};
+
+enum StoreCode : int {
+  ST_PLAIN = 0,     // d0[idx] = v
+  ST_FILTER,        // d0[idx] = scale v exp(-filter K^2)                 quasi.cpp:1260-1261
+  ST_UPDATE,        // semi-implicit AB2/CN update of psihat              quasi.cpp:1949-1951
+  ST_UPDATE_STB,    // CN update of psihat (stb)                          quasi-stb.cpp:1954-1956
+};
+
+struct LoadOp {
+  int code;
+  int axis;      // component c for GRAD/LAPGRAD/STRESS/VEL
+  int flag;      // DIVV: active; NONLIN: pressure term active; GRAD: source is double
+  int n0;        // VEL: slab size of the P-rank halo compat map (0 = exact)
+  const void* a[14];
+};
+
+struct StoreOp {
+  int code;
+  int first;           // UPDATE: first step (N_old = N)
+  void* d[3];          // outputs
+  const void* s[2];    // extra inputs
+  const double* dc;    // mixed precision: FP64 sum of the forward input (DC override), or null
+  double* dc_out;      // mixed precision: FP64 (0,0,0) mode written here, or null
+  const double* dc_old;// UPDATE: FP64 N_old(0,0,0) in mixed precision, or null
+};
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+template <typename T>
+__device__ __forceinline__ double rd(const void* p, long idx) {
+  return (double)((const T*)p)[idx];
+}
+
+// Shifted read along axis c used by the gradient stencils: src[m+1] or 0 at the end.
+template <typename T>
+__device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Waves& W, int i,
+                                           int j, int k, long idx) {
+  const long nyz = (long)G.ny * G.nz;
+  double s;
+  long o;
+  if (L.axis == 0) {
+    if (i >= G.nx - 1) return 0.0;
+    s = W.sx[i]; o = nyz;
+  } else if (L.axis == 1) {
+    if (j >= G.ny - 1) return 0.0;
+    s = W.sy[j]; o = G.nz;
+  } else {
+    if (k >= G.nz - 1) return 0.0;
+    s = W.sz[k]; o = 1;
+  }
+  double v = L.flag ? ((const double*)L.a[0])[idx + o] : (double)((const T*)L.a[0])[idx + o];
+  return -s * v;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Waves& W,
+                                       const Geo& G, int i, int j, int k, long idx) {
+  switch (L.code) {
+    case LD_PLAIN:
+      return ((const T*)L.a[0])[idx];
+    case LD_PLAIN_D:
+      return (T)((const double*)L.a[0])[idx];
+    case LD_ZERO:
+      return (T)0;
+    case LD_COEF_F:
+    case LD_COEF_G: {
+      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      const double fh = P.alpha * (P.q02 - mq2) * ((const double*)L.a[0])[idx];
+      return (T)(L.code == LD_COEF_F ? fh : -mq2 * fh);
+    }
+    case LD_GRAD:
+      return (T)grad_hat<T>(L, G, W, i, j, k, idx);
+    case LD_LAPGRAD: {
+      double m;
+      if (L.axis == 0) m = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      else if (L.axis == 1) m = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
+      else m = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]);
+      return (T)(-m * grad_hat<T>(L, G, W, i, j, k, idx));
+    }
+    case LD_SH: {
+      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
+      return (T)(-alin * ((const double*)L.a[0])[idx]);
+    }
+    case LD_RHOSTAR: {
+      const T ps = ((const T*)L.a[0])[idx], gx = ((const T*)L.a[1])[idx],
+              gy = ((const T*)L.a[2])[idx], gz = ((const T*)L.a[3])[idx];
+      const T r = (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
+      return r;
+    }
+    case LD_DIVV: {
+      if (!L.flag) return (T)0;
+      const T r = ((const T*)L.a[0])[idx], ro = ((const T*)L.a[1])[idx];
+      T t = (r - ro) / (T)P.dt + ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
+      t = t + ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
+      t = t + ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
+      return -t / ((T)0.5 * (r + ro));
+    }
+    case LD_STRESS: {
+      // a: rho, G, F, g_c, l_c
+      const T r = ((const T*)L.a[0])[idx];
+      const T gg = r * ((const T*)L.a[1])[idx];
+      const T rf = r * ((const T*)L.a[2])[idx];
+      return gg * ((const T*)L.a[3])[idx] - rf * ((const T*)L.a[4])[idx];
+    }
+    case LD_PRESSURE: {
+      // a: Sx, Sy, Sz (forward DST outputs); flag: stb gating i*j*k > 0
+      const long nyz = (long)G.ny * G.nz;
+      const bool on = L.flag ? ((long)i * j * k > 0) : (i + j + k != 0);
+      if (!on) return (T)0;
+      const double a = i > 0 ? rd<T>(L.a[0], idx - nyz) : 0.0;
+      const double b = j > 0 ? rd<T>(L.a[1], idx - G.nz) : 0.0;
+      const double c = k > 0 ? rd<T>(L.a[2], idx - 1) : 0.0;
+      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      return (T)(P.scale * (-(W.qx[i] * a + W.qy[j] * b + W.qz[k] * c) / mq2));
+    }
+    case LD_VEL: {
+      // a: Sx, Sy, Sz, Dhat (may be null).  Global-index form of quasi.cpp:1425-1872.
+      const long nyz = (long)G.ny * G.nz;
+      const void *Sx = L.a[0], *Sy = L.a[1], *Sz = L.a[2], *Dh = L.a[3];
+      const double nu = P.nu;
+      if (L.axis == 0) {
+        if (i >= G.nx - 1) return (T)0;
+        const double m2 = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+        const double cm1 = P.scale / (nu * m2), cm2 = W.sx[i] / m2;
+        const double sx = rd<T>(Sx, idx);
+        const double syv = j > 0 ? rd<T>(Sy, idx + nyz - G.nz) : 0.0;
+        const double szv = k > 0 ? rd<T>(Sz, idx + nyz - 1) : 0.0;
+        const double dot = W.sx[i] * sx + W.qy[j] * syv + W.qz[k] * szv;
+        const double dd = Dh ? rd<T>(Dh, idx + nyz) : 0.0;
+        return (T)(cm1 * (sx - cm2 * dot + nu * W.sx[i] * dd));
+      }
+      if (j >= G.ny - 1 || k >= G.nz - 1) return (T)0;
+      // x-neighbour slab of S_x: i-1, or the P-rank halo slab (quasi.cpp:1639-1650)
+      int ih = i - 1;
+      if (L.n0 > 0 && i > 0 && (i % L.n0) == 0) ih = i - L.n0;
+      const long dx = (long)(ih - i) * nyz;
+      if (L.axis == 1) {
+        const double m2 = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
+        const double sy = rd<T>(Sy, idx);
+        const double sxa = i > 0 ? rd<T>(Sx, idx + dx + G.nz) : 0.0;
+        const double szv = k > 0 ? rd<T>(Sz, idx + G.nz - 1) : 0.0;
+        const double dot = W.qx[i] * sxa + W.sy[j] * sy + W.qz[k] * szv;
+        const double dd = Dh ? rd<T>(Dh, idx + G.nz) : 0.0;
+        return (T)((P.scale / (nu * m2)) * (sy - (W.sy[j] / m2) * dot + nu * W.sy[j] * dd));
+      }
+      const double m2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]);
+      const double sz = rd<T>(Sz, idx);
+      const double sxb = i > 0 ? rd<T>(Sx, idx + dx + 1) : 0.0;
+      const double syv = j > 0 ? rd<T>(Sy, idx - G.nz + 1) : 0.0;
+      const double dot = W.qx[i] * sxb + W.qy[j] * syv + W.sz[k] * sz;
+      const double dd = Dh ? rd<T>(Dh, idx + 1) : 0.0;
+      return (T)((P.scale / (nu * m2)) * (sz - (W.sz[k] / m2) * dot + nu * W.sz[k] * dd));
+    }
+    case LD_SCALE:
+      return (T)(P.scale * rd<T>(L.a[0], idx));
+    case LD_NONLIN: {
+      // a: rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
+      const T r = ((const T*)L.a[0])[idx], ps = ((const T*)L.a[1])[idx];
+      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+      T v = r * ((T)P.eps * ps - ((const T*)L.a[2])[idx] + (T)P.beta * ps3 - (T)P.gamma * ps5) -
+            r * ((const T*)L.a[3])[idx];
+      v = v - ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
+      v = v - ((const T*)L.a[5])[idx] * ((const T*)L.a[8])[idx];
+      v = v - ((const T*)L.a[6])[idx] * ((const T*)L.a[9])[idx];
+      if (L.flag) {
+        const T rx = ((const T*)L.a[11])[idx], ry = ((const T*)L.a[12])[idx],
+                rz = ((const T*)L.a[13])[idx];
+        v = v + ((const T*)L.a[10])[idx] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
+                    (T)P.rho_s2;
+      }
+      return v;
+    }
+    case LD_STRESS_STB: {
+      // a: sh, psi, g_c ; rho = 1
+      const T ps = ((const T*)L.a[1])[idx];
+      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+      const T mu = ((const T*)L.a[0])[idx] - (T)P.beta * ps3 + (T)P.gamma * ps5;
+      return mu * ((const T*)L.a[2])[idx];
+    }
+    case LD_NONLIN_STB: {
+      // a: psi, vx, vy, vz, gx, gy, gz
+      const T ps = ((const T*)L.a[0])[idx];
+      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+      T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
+      if (P.hydro == 1) {
+        v = v - ((const T*)L.a[1])[idx] * ((const T*)L.a[4])[idx];
+        v = v - ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
+        v = v - ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
+      }
+      return v;
+    }
+    default:
+      return (T)0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_eval(const StoreOp& S, const Phys& P, const Waves& W, int i,
+                                           int j, int k, long idx, T v) {
+  switch (S.code) {
+    case ST_PLAIN:
+      ((T*)S.d[0])[idx] = v;
+      return;
+    case ST_FILTER: {
+      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      double raw = (double)v;
+      if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
+      const double r = P.scale * raw * exp(-P.filter * mq2);
+      ((T*)S.d[0])[idx] = (T)r;
+      if (S.dc_out && idx == 0) S.dc_out[0] = r;
+      return;
+    }
+    case ST_UPDATE:
+    case ST_UPDATE_STB: {
+      // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out)
+      // s0: psihat (double, in)
+      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+      const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
+      double nh = (double)v;
+      double nold;
+      if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
+      if (S.first) nold = nh;
+      else if (S.dc_old && idx == 0) nold = S.dc_old[0];
+      else nold = (double)((const T*)S.d[1])[idx];
+      const double ph = ((const double*)S.s[0])[idx];
+      double* pho = (double*)S.d[0];
+      double nw;
+      if (S.code == ST_UPDATE) {
+        const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
+        nw = (c1 * ph - 0.5 * pho[idx] + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
+      } else {
+        const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
+        nw = (c1 * ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
+      }
+      pho[idx] = nw;
+      ((T*)S.d[1])[idx] = (T)nh;
+      if (S.dc_out && idx == 0) S.dc_out[0] = nh;
+      return;
+    }
+    default:
+      return;
+  }
+}
+
+}  // namespace qsm

@@ -1,0 +1,156 @@
+"""GPU parity: libqsm (HIP, through the C ABI) against the CPU oracle.
+
+Transforms: every r2r kind along every axis for every compiled pencil length,
+FP64 to 1e-12 and FP32 to 2e-6 relative L2.  Step: every intermediate field
+of the reference step after one step, and psi/rho/v after several steps,
+FP64 to 1e-10 relative L2 (the drop-in tolerance), including the P-rank halo
+compat mode and the D / pressure branches (divv_switch lowered so they run).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+KINDS = (4, 5, 8, 9)  # REDFT01, REDFT10, RODFT01, RODFT10
+LENGTHS = (8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024)
+
+
+def _shape_for(n, axis):
+    other = {0: (n, 8, 32), 1: (8, n, 32), 2: (8, 16, n)}
+    return other[axis]
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+@pytest.mark.parametrize("axis", (0, 1, 2))
+def test_r2r_axis_fp64(qsm_mod, oracle_mod, n, axis):
+    shape = _shape_for(n, axis)
+    rng = np.random.default_rng(n * 10 + axis)
+    x = rng.standard_normal(shape)
+    for kind in KINDS:
+        kinds = [5, 4, 5]
+        kinds[axis] = kind
+        print(f"r2r fp64 n={n} axis={axis} kinds={kinds}", flush=True)
+        got = qsm_mod.Plan(shape, kinds, qsm_mod.FP64)(x)
+        ref = oracle_mod.r2r_3d(x, kinds)
+        assert rel_l2(got, ref) < 1e-12, (n, axis, kind, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+def test_r2r_fp32(qsm_mod, oracle_mod, n):
+    shape = _shape_for(n, 0)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(shape)
+    for kinds in ([5, 9, 4], [8, 4, 9], [9, 8, 5]):
+        print(f"r2r fp32 n={n} kinds={kinds}", flush=True)
+        got = qsm_mod.Plan(shape, kinds, qsm_mod.MIXED)(x.astype(np.float32))
+        ref = oracle_mod.r2r_3d(x.astype(np.float32).astype(np.float64), kinds)
+        assert rel_l2(got, ref) < 2e-6, (n, kinds, rel_l2(got, ref))
+
+
+def test_r2r_mixed_shape_roundtrip(qsm_mod):
+    shape = (48, 96, 64)
+    x = np.random.default_rng(7).standard_normal(shape)
+    f = qsm_mod.Plan(shape, [9, 5, 9], qsm_mod.FP64)
+    b = qsm_mod.Plan(shape, [8, 4, 8], qsm_mod.FP64)
+    y = b(f(x)) / (8.0 * shape[0] * shape[1] * shape[2])
+    assert rel_l2(y, x) < 1e-13
+
+
+def _pair(qsm, oracle, name="interface64", shape=None, prec=0, **over):
+    import ics
+    d = ics.params_dict(name, **over)
+    if shape is not None:
+        d["nx"], d["ny"], d["nz"] = shape
+    if "compat_ranks" not in over and d["compat_ranks"] and d["nx"] % d["compat_ranks"]:
+        d["compat_ranks"] = 0
+    qp = qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_})
+    op = oracle.make_params(**d)
+    psi = ics.workload_psi(name, shape=(d["nx"], d["ny"], d["nz"]))
+    sim = qsm.Sim(qp, prec)
+    sim.set_psi(psi)
+    orc = oracle.State(op)
+    orc.set_psi(psi)
+    return sim, orc
+
+
+STEP_FIELDS = ["F", "G", "GX", "GY", "GZ", "RX", "RY", "RZ", "LX", "LY", "LZ", "RHOHAT", "RHO",
+               "DHAT", "SX", "SY", "SZ", "P", "VX", "VY", "VZ", "DIVV", "NHAT", "PSIHAT",
+               "PSIHAT_OLD", "PSI", "PSI_OLD"]
+
+
+def _cmp_fields(sim, orc, names, tol, tag):
+    for f in names:
+        a = sim.field(f)
+        b = orc.field(f)
+        if f == "G" and orc.params.variant == 0:
+            # the reference leaves rho*G in this array (quasi.cpp:1305); the
+            # library keeps G and forms rho*G where it reads it
+            a = a * sim.field("RHO")
+        if not np.any(b):
+            assert not np.any(a), (tag, f)
+            continue
+        e = rel_l2(a, b)
+        print(f"{tag} {f}: rel_l2 {e:.2e}", flush=True)
+        assert e < tol, (tag, f, e)
+
+
+def test_setup_matches(qsm_mod, oracle_mod):
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=(32, 32, 32))
+    _cmp_fields(sim, orc, ["PSIHAT", "PSI"], 1e-13, "setup")
+
+
+@pytest.mark.parametrize("compat", (0, 4))
+def test_one_step_all_fields_fp64(qsm_mod, oracle_mod, compat):
+    # divv_switch = 0 makes the divergence and pressure branches live at step 2
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=(32, 48, 64), compat_ranks=compat, divv_switch=0)
+    for s in range(3):
+        sim.step(1)
+        orc.step(1)
+        _cmp_fields(sim, orc, STEP_FIELDS, 1e-10, f"compat{compat} step{s + 1}")
+
+
+def test_interface64_fp64_100_steps(qsm_mod, oracle_mod):
+    """64^3, 100 steps, FP64, P = 4 halo compat: the reference's mpirun -np 4 run."""
+    sim, orc = _pair(qsm_mod, oracle_mod)
+    sim.step(100)
+    orc.step(100)
+    for f in ("PSI", "RHO", "VX", "VY", "VZ"):
+        e = rel_l2(sim.field(f), orc.field(f))
+        print(f"interface64 step100 {f}: {e:.2e}", flush=True)
+        assert e < 1e-10, (f, e)
+    dg, do = sim.diag(), orc.diag()
+    for k in ("L1", "mass", "energy"):
+        assert abs(dg[k] - do[k]) <= 1e-10 * abs(do[k]), (k, dg[k], do[k])
+
+
+def test_stb_fp64(qsm_mod, oracle_mod):
+    import ics
+    qp = qsm_mod.default_params(qsm_mod.STB, nx=32, ny=32, nz=64, eps=-0.1, nu=0.5)
+    d = qp.as_dict()
+    op = oracle_mod.make_params(**d)
+    psi = ics.stb_psi((32, 32, 64), qi=d["growth_q"])
+    sim = qsm_mod.Sim(qp, qsm_mod.FP64)
+    sim.set_psi(psi)
+    orc = oracle_mod.State(op)
+    orc.set_psi(psi)
+    for s in range(4):
+        sim.step(1)
+        orc.step(1)
+        _cmp_fields(sim, orc, ["GX", "GY", "GZ", "SX", "SY", "SZ", "VX", "VY", "VZ", "NHAT",
+                               "PSIHAT", "PSI"], 1e-10, f"stb step{s + 1}")
+    g = sim.diag()["growth"]
+    assert abs(g - orc.growth) <= 1e-8 * max(1.0, abs(orc.growth)), (g, orc.growth)
+
+
+def test_mixed_vs_oracle(qsm_mod, oracle_mod):
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=(64, 64, 64), prec=1, divv_switch=5)
+    sim.step(20)
+    orc.step(20)
+    for f in ("PSI", "RHO"):
+        e = rel_l2(sim.field(f), orc.field(f))
+        print(f"mixed step20 {f}: {e:.2e}", flush=True)
+        assert e < 1e-5, (f, e)
+    dg, do = sim.diag(), orc.diag()
+    assert abs(dg["mass"] - do["mass"]) <= 1e-7 * abs(do["mass"])
