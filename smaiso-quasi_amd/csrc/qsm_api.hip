@@ -178,7 +178,8 @@ extern "C" int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void
     A.st.d[0] = d_out;
     A.tw = p->tw[ax];
     A.dst = kind_dst(p->kind[ax]);
-    hipError_t e = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), A, nullptr, s);
+    hipError_t e = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), LD_PLAIN, ST_PLAIN, A,
+                               nullptr, s);
     if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch failed: %s", hipGetErrorString(e));
   }
   return QSM_OK;
@@ -274,16 +275,19 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
     case LD_PLAIN: rd = e; break;
     case LD_PLAIN_D: case LD_COEF_F: case LD_COEF_G: case LD_SH: rd = 8; break;
     case LD_ZERO: rd = 0; break;
-    case LD_GRAD: case LD_LAPGRAD: rd = L.flag ? 8 : e; break;
+    case LD_GRAD: rd = e; break;
+    case LD_GRAD_D: case LD_LAPGRAD: rd = 8; break;
     case LD_RHOSTAR: rd = 4 * e; break;
-    case LD_DIVV: rd = L.flag ? 8 * e : 0; break;
+    case LD_DIVV: rd = 8 * e; break;
     case LD_STRESS: rd = 5 * e; break;
     case LD_PRESSURE: rd = 3 * e; break;
-    case LD_VEL: rd = (L.a[3] ? 4 : 3) * e; break;
+    case LD_VEL0: case LD_VEL1: case LD_VEL2: rd = (L.a[3] ? 4 : 3) * e; break;
     case LD_SCALE: rd = e; break;
-    case LD_NONLIN: rd = (L.flag ? 14 : 10) * e; break;
+    case LD_NONLIN: rd = 10 * e; break;
+    case LD_NONLIN_P: rd = 14 * e; break;
     case LD_STRESS_STB: rd = 3 * e; break;
-    case LD_NONLIN_STB: rd = (h->p.hydro == 1 ? 7 : 1) * e; break;
+    case LD_NONLIN_STB: rd = 7 * e; break;
+    case LD_NONLIN_STB0: rd = e; break;
     default: rd = e;
   }
   switch (S.code) {
@@ -309,7 +313,7 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   Timer& T = h->timer;
   hipEvent_t ea = nullptr, eb = nullptr;
   if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
-  hipError_t e = launch_pass(h->fp32, n, axis, inv, A, nullptr, h->stream);
+  hipError_t e = launch_pass(h->fp32, n, axis, inv, L.code, S.code, A, nullptr, h->stream);
   if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch (axis %d, n %d): %s", axis, n, hipGetErrorString(e));
   if (T.on) {
     hipEventRecord(eb, h->stream);
@@ -377,11 +381,11 @@ int quasi_step(qsm_handle_s* h) {
 
   // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_GRAD_D; L.axis = c; L.a[0] = h->psihat;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 0; L.a[0] = F(h, F_RHOHAT);
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = F(h, F_RHOHAT);
     if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.a[0] = h->psihat;
     if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
   }
 
@@ -394,7 +398,7 @@ int quasi_step(qsm_handle_s* h) {
   if ((rc = inv3(h, -1, plain_load(F(h, F_RHOHAT)), rho))) return rc;
 
   // S4: divergence (quasi.cpp:1273-1293)
-  L = LoadOp{}; L.code = LD_DIVV; L.flag = n > p.divv_switch;
+  L = LoadOp{}; L.code = n > p.divv_switch ? LD_DIVV : LD_ZERO;
   L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
   L.a[5] = F(h, F_RX); L.a[6] = F(h, F_RY); L.a[7] = F(h, F_RZ);
   if ((rc = fwd3(h, -1, L, F(h, F_DHAT), plain_store(F(h, F_DHAT)), -1))) return rc;
@@ -407,14 +411,14 @@ int quasi_step(qsm_handle_s* h) {
   }
 
   // S6: pressure (quasi.cpp:1336-1418)
-  L = LoadOp{}; L.code = LD_PRESSURE; L.flag = 0;
+  L = LoadOp{}; L.code = LD_PRESSURE;
   L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
   if ((rc = inv3(h, -1, L, F(h, F_P)))) return rc;
 
   // S7: velocity projection (quasi.cpp:1425-1884)
   const int n0 = p.compat_ranks > 0 ? p.nx / p.compat_ranks : 0;
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_VEL; L.axis = c; L.n0 = n0;
+    L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
     L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = F(h, F_DHAT);
     if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
   }
@@ -424,7 +428,7 @@ int quasi_step(qsm_handle_s* h) {
   if ((rc = inv3(h, -1, L, F(h, F_DIVV)))) return rc;
 
   // S9 + S10: N, its transform, and the update fused into the last pass
-  L = LoadOp{}; L.code = LD_NONLIN; L.flag = n > p.divv_switch + 1;
+  L = LoadOp{}; L.code = n > p.divv_switch + 1 ? LD_NONLIN_P : LD_NONLIN;
   L.a[0] = rho; L.a[1] = psi; L.a[2] = F(h, F_F); L.a[3] = F(h, F_G);
   L.a[4] = F(h, F_VX); L.a[5] = F(h, F_VY); L.a[6] = F(h, F_VZ);
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
@@ -452,7 +456,7 @@ int stb_step(qsm_handle_s* h) {
   L.code = LD_SH; L.a[0] = h->psihat;
   if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.flag = 1; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_GRAD_D; L.axis = c; L.a[0] = h->psihat;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
   }
   for (int c = 0; c < 3; ++c) {
@@ -462,11 +466,11 @@ int stb_step(qsm_handle_s* h) {
   }
   const int n0 = h->p.compat_ranks > 0 ? h->p.nx / h->p.compat_ranks : 0;
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_VEL; L.axis = c; L.n0 = n0;
+    L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
     L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = nullptr;
     if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
   }
-  L = LoadOp{}; L.code = LD_NONLIN_STB;
+  L = LoadOp{}; L.code = h->p.hydro == 1 ? LD_NONLIN_STB : LD_NONLIN_STB0;
   L.a[0] = psi; L.a[1] = F(h, F_VX); L.a[2] = F(h, F_VY); L.a[3] = F(h, F_VZ);
   L.a[4] = F(h, F_GX); L.a[5] = F(h, F_GY); L.a[6] = F(h, F_GZ);
   StoreOp S{}; S.code = ST_UPDATE_STB; S.first = n == 1;
@@ -557,7 +561,8 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     return fail(QSM_ERR_ARG, "qsm_create: dt, nu, q0 and nw must be positive");
   const int nn[3] = {p.nx, p.ny, p.nz};
   for (int a = 0; a < 3; ++a)
-    if (!pass_supported(nn[a])) return fail(QSM_ERR_UNSUPPORTED, "qsm_create: no pass kernel for length %d", nn[a]);
+    if (!step_supported(nn[a]))
+      return fail(QSM_ERR_UNSUPPORTED, "qsm_create: no step kernels for length %d (compiled: 32, 48, 64, 96, 128, 256, 512, 1024)", nn[a]);
 
   auto* h = new qsm_handle_s;
   h->p = p;

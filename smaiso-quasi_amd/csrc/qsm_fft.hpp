@@ -21,9 +21,10 @@
 // store (inverse).  These are FFTW's unnormalised definitions: a forward and
 // an inverse pass along one axis multiply by 2N.
 //
-// The M-point FFT is a mixed-radix Stockham autosort in LDS (radices 2,3,4,8,16),
-// E elements per thread, M/E threads per pencil, with twiddles read from a
-// table of e^{-2 pi i t/(4N)}, t < 4N, built on the host in FP64.
+// The M-point FFT is a mixed-radix Stockham autosort (radices 2,3,4,8,16)
+// with E elements per thread and M/E threads per pencil; values move between
+// stages through LDS, twiddles come from a quarter-turn table of
+// e^{-2 pi i t/(4N)} built on the host in FP64 and staged in LDS.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -147,60 +148,130 @@ __device__ __forceinline__ void dft(cx<T>* v) {
   else static_assert(R == 2, "unsupported radix");
 }
 
+// ---------------- twiddles ---------------------------------------------------
+// tq: LDS copy of e^{-2 pi i t/(4N)} for t in [0, N] (a quarter turn).  Any
+// e^{-2 pi i u/(4N)}, u in [0, 4N), is tq[u mod N] times (-i)^(u div N).
+template <typename T, int N>
+__device__ __forceinline__ cx<T> twq(const cx<T>* tq, int u) {
+  // (-i)^q (x + i y): q=0 (x, y), q=1 (y, -x), q=2 (-x, -y), q=3 (-y, x).
+  // Selects, not branches: the quadrant differs from lane to lane.
+  const int q = u / N, r = u - q * N;
+  const cx<T> w = tq[r];
+  const bool sw = (q & 1) != 0;
+  const T a = sw ? w.y : w.x, b = sw ? w.x : w.y;
+  const T sa = (q & 2) ? (T)-1 : (T)1;
+  const T sb = ((q ^ (q >> 1)) & 1) ? (T)-1 : (T)1;
+  return cx<T>{sa * a, sb * b};
+}
+
 // ---------------- Stockham stages ------------------------------------------
-// buf: this pencil's complex array (M entries).  t: thread index within the
-// pencil, TP = M/E threads.  tw: e^{-2 pi i u/(4N)} table, TWM = 4N/M.
-template <typename T, int M, int E, int R, int NS, int S, int TWM>
-__device__ __forceinline__ void stockham_stage(cx<T>* buf, int t, const cx<T>* __restrict__ tw) {
-  constexpr int TP = M / E;
-  constexpr int NB = E / R;
-  static_assert(E % R == 0, "E must be a multiple of every radix");
-  cx<T> v[NB][R];
+// The M-point FFT of TK pencils lives in LDS as buf[m * TKP + p] (complex
+// slot m of pencil p): the lane-varying index in every access is the pencil
+// p, so gathers and scatters of any stage hit consecutive addresses.
+// A thread (p, t), t < TP = M/E, holds E values: at stage (radix R, stride NS)
+// it owns butterflies j = t + b TP, b < E/R, whose inputs are slots
+// j + r M/R and whose outputs go to slots (j/NS) NS R + j%NS + r NS.
+template <int R, int... RS>
+struct First {
+  static constexpr int value = R;
+};
+
+template <int... Rs>
+struct Radices {};
+
+template <class RL>
+struct Last;
+template <int R>
+struct Last<Radices<R>> {
+  static constexpr int value = R;
+};
+template <int R, int R2, int... RS>
+struct Last<Radices<R, R2, RS...>> {
+  static constexpr int value = Last<Radices<R2, RS...>>::value;
+};
+
+template <typename T, int M, int E, int TKP, int R>
+__device__ __forceinline__ void gather(cx<T>* v, const cx<T>* buf, int p, int t) {
+  constexpr int TP = M / E, NB = E / R;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * TP;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * (M / R)];
-    if constexpr (NS > 1) {
-      const int jm = j % NS;
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const int u = jm * r * (M / (NS * R));
-        cx<T> w = tw[u * TWM];
-        if (S > 0) w.y = -w.y;
-        v[b][r] = cmul(v[b][r], w);
-      }
-    }
-    dft<R, S>(v[b]);
+    for (int r = 0; r < R; ++r) v[b * R + r] = buf[(j + r * (M / R)) * TKP + p];
   }
-  __syncthreads();
+}
+
+template <typename T, int M, int E, int TKP, int R, int NS>
+__device__ __forceinline__ void scatter(const cx<T>* v, cx<T>* buf, int p, int t) {
+  constexpr int TP = M / E, NB = E / R;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * TP;
     const int d = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[d + r * NS] = v[b][r];
+    for (int r = 0; r < R; ++r) buf[(d + r * NS) * TKP + p] = v[b * R + r];
   }
-  __syncthreads();
 }
 
-template <typename T, int M, int E, int S, int TWM, int NS, int R, int... RS>
-__device__ __forceinline__ void stockham_run(cx<T>* buf, int t, const cx<T>* __restrict__ tw) {
-  stockham_stage<T, M, E, R, NS, S, TWM>(buf, t, tw);
-  if constexpr (sizeof...(RS) > 0) stockham_run<T, M, E, S, TWM, NS * R, RS...>(buf, t, tw);
+// twiddle + radix-R DFT on the thread's registers; S = -1 forward, +1 inverse
+template <typename T, int N, int M, int E, int R, int NS, int S>
+__device__ __forceinline__ void stage_compute(cx<T>* v, int t, const cx<T>* tq) {
+  constexpr int TP = M / E, NB = E / R;
+  static_assert(E % R == 0, "E must be a multiple of every radix");
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if constexpr (NS > 1) {
+      const int jm = (t + b * TP) % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx<T> w = twq<T, N>(tq, 8 * (jm * r * (M / (NS * R))));  // e^{-2 pi i u/M}, 4N = 8M
+        if (S > 0) w.y = -w.y;
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
+    }
+    dft<R, S>(v + b * R);
+  }
+}
+
+// Runs the stages from the one whose inputs are already in v.  Between stages
+// the values go through LDS.  LAST_TO_LDS: scatter the last stage's outputs to
+// LDS (natural order); otherwise they stay in v, value b*R+r at slot
+// t + b TP + r M/R.
+template <typename T, int N, int M, int E, int TKP, int S, bool LAST_TO_LDS, int NS, int R,
+          int... RS>
+__device__ __forceinline__ void fft_from_regs(cx<T>* v, cx<T>* buf, int p, int t,
+                                              const cx<T>* tq) {
+  stage_compute<T, N, M, E, R, NS, S>(v, t, tq);
+  if constexpr (sizeof...(RS) == 0) {
+    if constexpr (LAST_TO_LDS) {
+      __syncthreads();
+      scatter<T, M, E, TKP, R, NS>(v, buf, p, t);
+      __syncthreads();
+    }
+  } else {
+    __syncthreads();  // every thread has read this stage's inputs
+    scatter<T, M, E, TKP, R, NS>(v, buf, p, t);
+    __syncthreads();
+    gather<T, M, E, TKP, First<RS...>::value>(v, buf, p, t);
+    fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, NS * R, RS...>(v, buf, p, t, tq);
+  }
+}
+
+template <typename T, int N, int M, int E, int TKP, int S, bool LAST_TO_LDS, int... RS>
+__device__ __forceinline__ void run_fft(Radices<RS...>, cx<T>* v, cx<T>* buf, int p, int t,
+                                        const cx<T>* tq) {
+  fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, 1, RS...>(v, buf, p, t, tq);
 }
 
 // Radix plan per half-length M: E elements per thread and the radix list.
 template <int M>
 struct Plan;
-#define QSM_PLAN(M_, E_, ...)                                                            \
-  template <>                                                                            \
-  struct Plan<M_> {                                                                      \
-    static constexpr int E = E_;                                                         \
-    template <typename T, int S, int TWM>                                                \
-    __device__ static __forceinline__ void run(cx<T>* b, int t, const cx<T>* tw) {       \
-      stockham_run<T, M_, E_, S, TWM, 1, __VA_ARGS__>(b, t, tw);                         \
-    }                                                                                    \
+#define QSM_PLAN(M_, E_, ...)                        \
+  template <>                                        \
+  struct Plan<M_> {                                  \
+    static constexpr int E = E_;                     \
+    using R = Radices<__VA_ARGS__>;                  \
+    static constexpr int R1 = First<__VA_ARGS__>::value; \
   };
 QSM_PLAN(4, 4, 4)
 QSM_PLAN(8, 8, 8)

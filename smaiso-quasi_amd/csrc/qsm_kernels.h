@@ -8,6 +8,10 @@
 // product of the radices 2, 3, 4, 8, 16 (qsm_fft.hpp).
 #define QSM_FOR_EACH_N(X) \
   X(8) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+// Lengths that also carry the step's fused operators.
+#define QSM_IS_STEP_N(n)                                                                      \
+  ((n) == 32 || (n) == 48 || (n) == 64 || (n) == 96 || (n) == 128 || (n) == 256 || (n) == 512 || \
+   (n) == 1024)
 
 namespace qsm {
 
@@ -25,9 +29,13 @@ struct PassArgs {
 
 // One pass of a 3D r2r transform along `axis` (0 = x, 1 = y, 2 = z) of length n.
 // inv = REDFT01/RODFT01 (DCT-III/DST-III); otherwise REDFT10/RODFT10.
-hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, const PassArgs& A, long* nblocks,
-                       hipStream_t s);
+// ld / st: the LoadCode / StoreCode of the fused operators (A.ld.code and
+// A.st.code are ignored); combinations that are not compiled return
+// hipErrorInvalidValue.
+hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
+                       long* nblocks, hipStream_t s);
 bool pass_supported(int n);
+bool step_supported(int n);
 int pass_tile_k(int prec_fp32, int n, int axis);
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 

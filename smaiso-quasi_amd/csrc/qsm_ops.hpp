@@ -33,21 +33,26 @@ struct Phys {
 enum LoadCode : int {
   LD_PLAIN = 0,     // a0[idx] (field type T)
   LD_PLAIN_D,       // (T) a0[idx], a0 is double (psihat)
-  LD_ZERO,          // 0
+  LD_ZERO,          // 0 (divergence before divv_switch: the reference transforms zeros)
   LD_COEF_F,        // alpha (q0^2 - K^2) psihat                          quasi.cpp:1042
   LD_COEF_G,        // -K^2 alpha (q0^2 - K^2) psihat                     quasi.cpp:1044-1045
-  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1          quasi.cpp:1124-1171
-  LD_LAPGRAD,       // -K_c^2 (-s_c[m] src[m+1])                          quasi.cpp:1176-1191
+  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1 (src: T) quasi.cpp:1124-1171
+  LD_GRAD_D,        // same with a double source (psihat)
+  LD_LAPGRAD,       // -K_c^2 (-s_c[m] psihat[m+1])                       quasi.cpp:1176-1191
   LD_SH,            // -aLin psihat (stb)                                 quasi-stb.cpp:1023
   LD_RHOSTAR,       // kp sqrt(q0^2 psi^2 + |g|^2) + rho0                 quasi.cpp:1243-1247
   LD_DIVV,          // -((rho-rho_old)/dt + v.r)/(0.5(rho+rho_old))       quasi.cpp:1280-1285
   LD_STRESS,        // (rho G) g_c - (rho F) l_c                          quasi.cpp:1305-1317
   LD_PRESSURE,      // scale (-(q.S_shifted)/K^2)                         quasi.cpp:1385-1415
-  LD_VEL,           // Stokes/Leray projection of S                       quasi.cpp:1522-1872
+  LD_VEL0,          // Stokes/Leray projection of S, component x          quasi.cpp:1522-1634
+  LD_VEL1,          // component y                                        quasi.cpp:1681-1872
+  LD_VEL2,          // component z                                        quasi.cpp:1681-1872
   LD_SCALE,         // scale a0[idx]                                      quasi.cpp:1891-1896
-  LD_NONLIN,        // N of quasi.cpp:1914-1928
+  LD_NONLIN,        // N of quasi.cpp:1914-1923 (steps <= divv_switch+1)
+  LD_NONLIN_P,      // N with the pressure term of quasi.cpp:1925-1930
   LD_STRESS_STB,    // rho mu g_c, mu = sh - beta psi^3 + gamma psi^5     quasi-stb.cpp:1295-1307
-  LD_NONLIN_STB,    // beta psi^3 - gamma psi^5 - v.g                     quasi-stb.cpp:1903-1914
+  LD_NONLIN_STB,    // beta psi^3 - gamma psi^5 - v.g (hydro = 1)         quasi-stb.cpp:1903-1914
+  LD_NONLIN_STB0,   // beta psi^3 - gamma psi^5 (hydro = 0)               quasi-stb.cpp:1915-1919
 };
 
 enum StoreCode : int {
@@ -59,9 +64,9 @@ enum StoreCode : int {
 
 struct LoadOp {
   int code;
-  int axis;      // component c for GRAD/LAPGRAD/STRESS/VEL
-  int flag;      // DIVV: active; NONLIN: pressure term active; GRAD: source is double
-  int n0;        // VEL: slab size of the P-rank halo compat map (0 = exact)
+  int axis;      // component c for GRAD/GRAD_D/LAPGRAD
+  int flag;      // unused (reserved)
+  int n0;        // VEL0-2: slab size of the P-rank halo compat map (0 = exact)
   const void* a[14];
 };
 
@@ -82,215 +87,206 @@ __device__ __forceinline__ double rd(const void* p, long idx) {
   return (double)((const T*)p)[idx];
 }
 
-// Shifted read along axis c used by the gradient stencils: src[m+1] or 0 at the end.
+// p[idx + off] if ok, else 0, without a branch: the load always reads a valid
+// address (idx itself when !ok) and the result is selected afterwards, so
+// the loads of one element can be issued together.
 template <typename T>
+__device__ __forceinline__ double rd_if(const void* p, long idx, long off, bool ok) {
+  const double v = (double)((const T*)p)[ok ? idx + off : idx];
+  return ok ? v : 0.0;
+}
+
+// Shifted read along axis c used by the gradient stencils: -s_c[m] src[m+1],
+// 0 at m = N_c - 1.  The axis is uniform for a launch, so the per-element
+// choices below are selects, not branches.
+template <typename S>
 __device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Waves& W, int i,
                                            int j, int k, long idx) {
   const long nyz = (long)G.ny * G.nz;
-  double s;
-  long o;
-  if (L.axis == 0) {
-    if (i >= G.nx - 1) return 0.0;
-    s = W.sx[i]; o = nyz;
-  } else if (L.axis == 1) {
-    if (j >= G.ny - 1) return 0.0;
-    s = W.sy[j]; o = G.nz;
-  } else {
-    if (k >= G.nz - 1) return 0.0;
-    s = W.sz[k]; o = 1;
-  }
-  double v = L.flag ? ((const double*)L.a[0])[idx + o] : (double)((const T*)L.a[0])[idx + o];
-  return -s * v;
+  const int c = L.axis;
+  const int m = c == 0 ? i : (c == 1 ? j : k);
+  const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
+  const long o = c == 0 ? nyz : (c == 1 ? (long)G.nz : 1L);
+  const double* st = c == 0 ? W.sx : (c == 1 ? W.sy : W.sz);
+  const bool ok = m < nc - 1;
+  return -st[m] * rd_if<S>(L.a[0], idx, o, ok);
 }
 
-template <typename T>
+template <typename T, int LD>
 __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Waves& W,
                                        const Geo& G, int i, int j, int k, long idx) {
-  switch (L.code) {
-    case LD_PLAIN:
-      return ((const T*)L.a[0])[idx];
-    case LD_PLAIN_D:
-      return (T)((const double*)L.a[0])[idx];
-    case LD_ZERO:
-      return (T)0;
-    case LD_COEF_F:
-    case LD_COEF_G: {
-      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      const double fh = P.alpha * (P.q02 - mq2) * ((const double*)L.a[0])[idx];
-      return (T)(L.code == LD_COEF_F ? fh : -mq2 * fh);
-    }
-    case LD_GRAD:
-      return (T)grad_hat<T>(L, G, W, i, j, k, idx);
-    case LD_LAPGRAD: {
-      double m;
-      if (L.axis == 0) m = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      else if (L.axis == 1) m = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
-      else m = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]);
-      return (T)(-m * grad_hat<T>(L, G, W, i, j, k, idx));
-    }
-    case LD_SH: {
-      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-      return (T)(-alin * ((const double*)L.a[0])[idx]);
-    }
-    case LD_RHOSTAR: {
-      const T ps = ((const T*)L.a[0])[idx], gx = ((const T*)L.a[1])[idx],
-              gy = ((const T*)L.a[2])[idx], gz = ((const T*)L.a[3])[idx];
-      const T r = (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
-      return r;
-    }
-    case LD_DIVV: {
-      if (!L.flag) return (T)0;
-      const T r = ((const T*)L.a[0])[idx], ro = ((const T*)L.a[1])[idx];
-      T t = (r - ro) / (T)P.dt + ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
-      t = t + ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
-      t = t + ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
-      return -t / ((T)0.5 * (r + ro));
-    }
-    case LD_STRESS: {
-      // a: rho, G, F, g_c, l_c
-      const T r = ((const T*)L.a[0])[idx];
-      const T gg = r * ((const T*)L.a[1])[idx];
-      const T rf = r * ((const T*)L.a[2])[idx];
-      return gg * ((const T*)L.a[3])[idx] - rf * ((const T*)L.a[4])[idx];
-    }
-    case LD_PRESSURE: {
-      // a: Sx, Sy, Sz (forward DST outputs); flag: stb gating i*j*k > 0
-      const long nyz = (long)G.ny * G.nz;
-      const bool on = L.flag ? ((long)i * j * k > 0) : (i + j + k != 0);
-      if (!on) return (T)0;
-      const double a = i > 0 ? rd<T>(L.a[0], idx - nyz) : 0.0;
-      const double b = j > 0 ? rd<T>(L.a[1], idx - G.nz) : 0.0;
-      const double c = k > 0 ? rd<T>(L.a[2], idx - 1) : 0.0;
-      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      return (T)(P.scale * (-(W.qx[i] * a + W.qy[j] * b + W.qz[k] * c) / mq2));
-    }
-    case LD_VEL: {
-      // a: Sx, Sy, Sz, Dhat (may be null).  Global-index form of quasi.cpp:1425-1872.
-      const long nyz = (long)G.ny * G.nz;
-      const void *Sx = L.a[0], *Sy = L.a[1], *Sz = L.a[2], *Dh = L.a[3];
-      const double nu = P.nu;
-      if (L.axis == 0) {
-        if (i >= G.nx - 1) return (T)0;
-        const double m2 = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-        const double cm1 = P.scale / (nu * m2), cm2 = W.sx[i] / m2;
-        const double sx = rd<T>(Sx, idx);
-        const double syv = j > 0 ? rd<T>(Sy, idx + nyz - G.nz) : 0.0;
-        const double szv = k > 0 ? rd<T>(Sz, idx + nyz - 1) : 0.0;
-        const double dot = W.sx[i] * sx + W.qy[j] * syv + W.qz[k] * szv;
-        const double dd = Dh ? rd<T>(Dh, idx + nyz) : 0.0;
-        return (T)(cm1 * (sx - cm2 * dot + nu * W.sx[i] * dd));
-      }
-      if (j >= G.ny - 1 || k >= G.nz - 1) return (T)0;
-      // x-neighbour slab of S_x: i-1, or the P-rank halo slab (quasi.cpp:1639-1650)
-      int ih = i - 1;
-      if (L.n0 > 0 && i > 0 && (i % L.n0) == 0) ih = i - L.n0;
-      const long dx = (long)(ih - i) * nyz;
-      if (L.axis == 1) {
-        const double m2 = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
-        const double sy = rd<T>(Sy, idx);
-        const double sxa = i > 0 ? rd<T>(Sx, idx + dx + G.nz) : 0.0;
-        const double szv = k > 0 ? rd<T>(Sz, idx + G.nz - 1) : 0.0;
-        const double dot = W.qx[i] * sxa + W.sy[j] * sy + W.qz[k] * szv;
-        const double dd = Dh ? rd<T>(Dh, idx + G.nz) : 0.0;
-        return (T)((P.scale / (nu * m2)) * (sy - (W.sy[j] / m2) * dot + nu * W.sy[j] * dd));
-      }
+  if constexpr (LD == LD_PLAIN) {
+    return ((const T*)L.a[0])[idx];
+  } else if constexpr (LD == LD_PLAIN_D) {
+    return (T)((const double*)L.a[0])[idx];
+  } else if constexpr (LD == LD_ZERO) {
+    return (T)0;
+  } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
+    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double fh = P.alpha * (P.q02 - mq2) * ((const double*)L.a[0])[idx];
+    return (T)(LD == LD_COEF_F ? fh : -mq2 * fh);
+  } else if constexpr (LD == LD_GRAD) {
+    return (T)grad_hat<T>(L, G, W, i, j, k, idx);
+  } else if constexpr (LD == LD_GRAD_D) {
+    return (T)grad_hat<double>(L, G, W, i, j, k, idx);
+  } else if constexpr (LD == LD_LAPGRAD) {
+    const int c = L.axis;
+    const double ax = c == 0 ? W.sx[i] : W.qx[i];
+    const double ay = c == 1 ? W.sy[j] : W.qy[j];
+    const double az = c == 2 ? W.sz[k] : W.qz[k];
+    const double m = sq(ax) + sq(ay) + sq(az);
+    return (T)(-m * grad_hat<double>(L, G, W, i, j, k, idx));
+  } else if constexpr (LD == LD_SH) {
+    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
+    return (T)(-alin * ((const double*)L.a[0])[idx]);
+  } else if constexpr (LD == LD_RHOSTAR) {
+    const T ps = ((const T*)L.a[0])[idx], gx = ((const T*)L.a[1])[idx],
+            gy = ((const T*)L.a[2])[idx], gz = ((const T*)L.a[3])[idx];
+    return (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
+  } else if constexpr (LD == LD_DIVV) {
+    // a: rho, rho_old, vx, vy, vz, rx, ry, rz
+    const T r = ((const T*)L.a[0])[idx], ro = ((const T*)L.a[1])[idx];
+    T t = (r - ro) / (T)P.dt + ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
+    t = t + ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
+    t = t + ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
+    return -t / ((T)0.5 * (r + ro));
+  } else if constexpr (LD == LD_STRESS) {
+    // a: rho, G, F, g_c, l_c
+    const T r = ((const T*)L.a[0])[idx];
+    const T gg = r * ((const T*)L.a[1])[idx];
+    const T rf = r * ((const T*)L.a[2])[idx];
+    return gg * ((const T*)L.a[3])[idx] - rf * ((const T*)L.a[4])[idx];
+  } else if constexpr (LD == LD_PRESSURE) {
+    // a: Sx, Sy, Sz (forward DST outputs)
+    const long nyz = (long)G.ny * G.nz;
+    const double a = rd_if<T>(L.a[0], idx, -nyz, i > 0);
+    const double b = rd_if<T>(L.a[1], idx, -(long)G.nz, j > 0);
+    const double c = rd_if<T>(L.a[2], idx, -1, k > 0);
+    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double pv = P.scale * (-(W.qx[i] * a + W.qy[j] * b + W.qz[k] * c) / mq2);
+    return (T)((i + j + k == 0) ? 0.0 : pv);
+  } else if constexpr (LD == LD_VEL0) {
+    // a: Sx, Sy, Sz, Dhat (may be null).  Global-index form of quasi.cpp:1522-1634.
+    const long nyz = (long)G.ny * G.nz;
+    const bool in = i < G.nx - 1;  // v_x = 0 at i = Nx-1
+    const double nu = P.nu;
+    const double m2 = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double cm1 = P.scale / (nu * m2), cm2 = W.sx[i] / m2;
+    const double sx = rd<T>(L.a[0], idx);
+    const double syv = rd_if<T>(L.a[1], idx, nyz - G.nz, in && j > 0);
+    const double szv = rd_if<T>(L.a[2], idx, nyz - 1, in && k > 0);
+    const double dot = W.sx[i] * sx + W.qy[j] * syv + W.qz[k] * szv;
+    const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, nyz, in) : 0.0;
+    const double vx = cm1 * (sx - cm2 * dot + nu * W.sx[i] * dd);
+    return (T)(in ? vx : 0.0);
+  } else if constexpr (LD == LD_VEL1 || LD == LD_VEL2) {
+    // quasi.cpp:1681-1872: zero at j = Ny-1 or k = Nz-1.  The S_x neighbour
+    // slab is i-1, or the P-rank halo slab (quasi.cpp:1639-1650).
+    const long nyz = (long)G.ny * G.nz;
+    const bool in = j < G.ny - 1 && k < G.nz - 1;
+    const double nu = P.nu;
+    int ih = i - 1;
+    if (L.n0 > 0 && i > 0 && (i % L.n0) == 0) ih = i - L.n0;
+    const long dx = (long)(ih - i) * nyz;
+    double vv;
+    if constexpr (LD == LD_VEL1) {
+      const double m2 = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
+      const double sy = rd<T>(L.a[1], idx);
+      const double sxa = rd_if<T>(L.a[0], idx, dx + G.nz, in && i > 0);
+      const double szv = rd_if<T>(L.a[2], idx, G.nz - 1, in && k > 0);
+      const double dot = W.qx[i] * sxa + W.sy[j] * sy + W.qz[k] * szv;
+      const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, G.nz, in) : 0.0;
+      vv = (P.scale / (nu * m2)) * (sy - (W.sy[j] / m2) * dot + nu * W.sy[j] * dd);
+    } else {
       const double m2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]);
-      const double sz = rd<T>(Sz, idx);
-      const double sxb = i > 0 ? rd<T>(Sx, idx + dx + 1) : 0.0;
-      const double syv = j > 0 ? rd<T>(Sy, idx - G.nz + 1) : 0.0;
+      const double sz = rd<T>(L.a[2], idx);
+      const double sxb = rd_if<T>(L.a[0], idx, dx + 1, in && i > 0);
+      const double syv = rd_if<T>(L.a[1], idx, 1 - (long)G.nz, in && j > 0);
       const double dot = W.qx[i] * sxb + W.qy[j] * syv + W.sz[k] * sz;
-      const double dd = Dh ? rd<T>(Dh, idx + 1) : 0.0;
-      return (T)((P.scale / (nu * m2)) * (sz - (W.sz[k] / m2) * dot + nu * W.sz[k] * dd));
+      const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, 1, in) : 0.0;
+      vv = (P.scale / (nu * m2)) * (sz - (W.sz[k] / m2) * dot + nu * W.sz[k] * dd);
     }
-    case LD_SCALE:
-      return (T)(P.scale * rd<T>(L.a[0], idx));
-    case LD_NONLIN: {
-      // a: rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
-      const T r = ((const T*)L.a[0])[idx], ps = ((const T*)L.a[1])[idx];
-      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-      T v = r * ((T)P.eps * ps - ((const T*)L.a[2])[idx] + (T)P.beta * ps3 - (T)P.gamma * ps5) -
-            r * ((const T*)L.a[3])[idx];
-      v = v - ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
-      v = v - ((const T*)L.a[5])[idx] * ((const T*)L.a[8])[idx];
-      v = v - ((const T*)L.a[6])[idx] * ((const T*)L.a[9])[idx];
-      if (L.flag) {
-        const T rx = ((const T*)L.a[11])[idx], ry = ((const T*)L.a[12])[idx],
-                rz = ((const T*)L.a[13])[idx];
-        v = v + ((const T*)L.a[10])[idx] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
-                    (T)P.rho_s2;
-      }
-      return v;
+    return (T)(in ? vv : 0.0);
+  } else if constexpr (LD == LD_SCALE) {
+    return (T)(P.scale * rd<T>(L.a[0], idx));
+  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P) {
+    // a: rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
+    const T r = ((const T*)L.a[0])[idx], ps = ((const T*)L.a[1])[idx];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    T v = r * ((T)P.eps * ps - ((const T*)L.a[2])[idx] + (T)P.beta * ps3 - (T)P.gamma * ps5) -
+          r * ((const T*)L.a[3])[idx];
+    v = v - ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
+    v = v - ((const T*)L.a[5])[idx] * ((const T*)L.a[8])[idx];
+    v = v - ((const T*)L.a[6])[idx] * ((const T*)L.a[9])[idx];
+    if constexpr (LD == LD_NONLIN_P) {
+      const T rx = ((const T*)L.a[11])[idx], ry = ((const T*)L.a[12])[idx],
+              rz = ((const T*)L.a[13])[idx];
+      v = v + ((const T*)L.a[10])[idx] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
+                  (T)P.rho_s2;
     }
-    case LD_STRESS_STB: {
-      // a: sh, psi, g_c ; rho = 1
-      const T ps = ((const T*)L.a[1])[idx];
-      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-      const T mu = ((const T*)L.a[0])[idx] - (T)P.beta * ps3 + (T)P.gamma * ps5;
-      return mu * ((const T*)L.a[2])[idx];
+    return v;
+  } else if constexpr (LD == LD_STRESS_STB) {
+    // a: sh, psi, g_c ; rho = 1
+    const T ps = ((const T*)L.a[1])[idx];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    const T mu = ((const T*)L.a[0])[idx] - (T)P.beta * ps3 + (T)P.gamma * ps5;
+    return mu * ((const T*)L.a[2])[idx];
+  } else if constexpr (LD == LD_NONLIN_STB || LD == LD_NONLIN_STB0) {
+    // a: psi, vx, vy, vz, gx, gy, gz
+    const T ps = ((const T*)L.a[0])[idx];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
+    if constexpr (LD == LD_NONLIN_STB) {
+      v = v - ((const T*)L.a[1])[idx] * ((const T*)L.a[4])[idx];
+      v = v - ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
+      v = v - ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
     }
-    case LD_NONLIN_STB: {
-      // a: psi, vx, vy, vz, gx, gy, gz
-      const T ps = ((const T*)L.a[0])[idx];
-      const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-      T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
-      if (P.hydro == 1) {
-        v = v - ((const T*)L.a[1])[idx] * ((const T*)L.a[4])[idx];
-        v = v - ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
-        v = v - ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
-      }
-      return v;
-    }
-    default:
-      return (T)0;
+    return v;
+  } else {
+    static_assert(LD == LD_PLAIN, "unknown load operator");
+    return (T)0;
   }
 }
 
-template <typename T>
+template <typename T, int ST>
 __device__ __forceinline__ void store_eval(const StoreOp& S, const Phys& P, const Waves& W, int i,
                                            int j, int k, long idx, T v) {
-  switch (S.code) {
-    case ST_PLAIN:
-      ((T*)S.d[0])[idx] = v;
-      return;
-    case ST_FILTER: {
-      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      double raw = (double)v;
-      if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
-      const double r = P.scale * raw * exp(-P.filter * mq2);
-      ((T*)S.d[0])[idx] = (T)r;
-      if (S.dc_out && idx == 0) S.dc_out[0] = r;
-      return;
+  if constexpr (ST == ST_PLAIN) {
+    ((T*)S.d[0])[idx] = v;
+  } else if constexpr (ST == ST_FILTER) {
+    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    double raw = (double)v;
+    if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
+    const double r = P.scale * raw * exp(-P.filter * mq2);
+    ((T*)S.d[0])[idx] = (T)r;
+    if (S.dc_out && idx == 0) S.dc_out[0] = r;
+  } else if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
+    // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out)
+    // s0: psihat (double, in)
+    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
+    double nh = (double)v;
+    if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
+    double nold;
+    if (S.first) nold = nh;
+    else if (S.dc_old && idx == 0) nold = S.dc_old[0];
+    else nold = (double)((const T*)S.d[1])[idx];
+    const double ph = ((const double*)S.s[0])[idx];
+    double* pho = (double*)S.d[0];
+    double nw;
+    if constexpr (ST == ST_UPDATE) {
+      const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
+      nw = (c1 * ph - 0.5 * pho[idx] + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
+    } else {
+      const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
+      nw = (c1 * ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
     }
-    case ST_UPDATE:
-    case ST_UPDATE_STB: {
-      // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out)
-      // s0: psihat (double, in)
-      const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-      const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-      double nh = (double)v;
-      double nold;
-      if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
-      if (S.first) nold = nh;
-      else if (S.dc_old && idx == 0) nold = S.dc_old[0];
-      else nold = (double)((const T*)S.d[1])[idx];
-      const double ph = ((const double*)S.s[0])[idx];
-      double* pho = (double*)S.d[0];
-      double nw;
-      if (S.code == ST_UPDATE) {
-        const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
-        nw = (c1 * ph - 0.5 * pho[idx] + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
-      } else {
-        const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
-        nw = (c1 * ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
-      }
-      pho[idx] = nw;
-      ((T*)S.d[1])[idx] = (T)nh;
-      if (S.dc_out && idx == 0) S.dc_out[0] = nh;
-      return;
-    }
-    default:
-      return;
+    pho[idx] = nw;
+    ((T*)S.d[1])[idx] = (T)nh;
+    if (S.dc_out && idx == 0) S.dc_out[0] = nh;
+  } else {
+    static_assert(ST == ST_PLAIN, "unknown store operator");
   }
 }
 

@@ -1,0 +1,31 @@
+// qsm_pass_inst.hip -- instantiates the pass kernels of one field precision
+// (QSM_T) and one axis (QSM_AX) for every compiled pencil length; the build
+// compiles this file once per (precision, axis) pair.
+#include "qsm_pass_impl.hpp"
+
+#ifndef QSM_T
+#error "QSM_T (float or double) must be defined"
+#endif
+#ifndef QSM_AX
+#error "QSM_AX (0, 1 or 2) must be defined"
+#endif
+
+#define QSM_CAT2(a, b, c) a##_##b##_##c
+#define QSM_CAT(a, b, c) QSM_CAT2(a, b, c)
+
+namespace qsm {
+
+hipError_t QSM_CAT(launch_pass, QSM_T, QSM_AX)(int n, bool inv, int ld, int st, const PassArgs& A,
+                                               long* nb, hipStream_t s) {
+  switch (n) {
+#define QSM_CASE(NN) \
+  case NN:           \
+    return dispatch_ops<QSM_T, QSM_AX, NN, QSM_IS_STEP_N(NN)>(inv, ld, st, A, nb, s);
+    QSM_FOR_EACH_N(QSM_CASE)
+#undef QSM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace qsm

@@ -103,8 +103,8 @@ def test_setup_matches(qsm_mod, oracle_mod):
 
 @pytest.mark.parametrize("compat", (0, 4))
 def test_one_step_all_fields_fp64(qsm_mod, oracle_mod, compat):
-    # divv_switch = 0 makes the divergence and pressure branches live at step 2
-    sim, orc = _pair(qsm_mod, oracle_mod, shape=(32, 48, 64), compat_ranks=compat, divv_switch=0)
+    # divv_switch = 1 makes the divergence branch live at step 2 and the pressure term at step 3
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=(32, 48, 64), compat_ranks=compat, divv_switch=1)
     for s in range(3):
         sim.step(1)
         orc.step(1)
