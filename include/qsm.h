@@ -142,6 +142,9 @@ int qsm_create(const qsm_params* p, int prec, qsm_handle* out);
 int qsm_set_psi(qsm_handle h, const double* psi);
 /* advance nsteps steps (quasi.cpp:1017-1965 per step) */
 int qsm_step(qsm_handle h, int nsteps);
+/* steps are replayed as captured HIP graphs (one per buffer/phase signature,
+ * captured on first use); enable = 0 launches every kernel eagerly */
+int qsm_set_graphs(qsm_handle h, int enable);
 /* run until L1 <= lim_l1 (checked every step_l1 steps, quasi.cpp:1014, 1972,
  * 2010) or max_steps; *steps_done receives the steps taken */
 int qsm_run_l1(qsm_handle h, int max_steps, int step_l1, double lim_l1, int* steps_done);

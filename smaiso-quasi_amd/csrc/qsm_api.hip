@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -236,7 +237,8 @@ struct qsm_handle_s {
   Geo geo{};
   Waves waves{};
   Phys phys{};
-  double* wtab = nullptr;     // 6 wavenumber tables
+  double* wtab = nullptr;     // 6 wavenumber tables + 3 filter factors
+  double* ones = nullptr;     // unit filter factors (setup scaling)
   const void* tw[3] = {nullptr, nullptr, nullptr};
   void* f[F_COUNT] = {};
   int psi_cur = F_PSI_A, psi_old = F_PSI_B, rho_cur = F_RHO_A, rho_old = F_RHO_B;
@@ -249,6 +251,11 @@ struct qsm_handle_s {
   bool ready = false;
   double growth = 0;
   Timer timer;
+  // hipGraph replay of whole steps, keyed by everything the step's kernel
+  // arguments depend on (ping-pong buffer state and phase flags)
+  bool use_graphs = true;
+  bool dry = false;  // sequence the step without launching (state bookkeeping only)
+  std::map<std::vector<long long>, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -310,6 +317,7 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   A.part = part;
   A.dst = dst ? 1 : 0;
   const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
+  if (h->dry) return QSM_OK;
   Timer& T = h->timer;
   hipEvent_t ea = nullptr, eb = nullptr;
   if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
@@ -352,7 +360,7 @@ int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreO
   double* part = dc_slot >= 0 ? h->part : nullptr;
   int rc = run_pass(h, 2, false, dst_axis == 2, L, plain_store(tmp), true, part);
   if (rc) return rc;
-  if (part) {
+  if (part && !h->dry) {
     const long nb = pass_blocks(h->fp32, h->p.nz, 2, h->geo);
     hipError_t e = launch_finish(part, nb, h->dsc + dc_slot, h->stream);
     if (e != hipSuccess) return fail(QSM_ERR_HIP, "dc sum: %s", hipGetErrorString(e));
@@ -512,7 +520,10 @@ void free_handle(qsm_handle_s* h) {
   if (h->psihat_old) hipFree(h->psihat_old);
   if (h->part) hipFree(h->part);
   if (h->dsc) hipFree(h->dsc);
+  for (auto& g : h->graphs) hipGraphExecDestroy(g.second);
+  h->graphs.clear();
   if (h->wtab) hipFree(h->wtab);
+  if (h->ones) hipFree(h->ones);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -577,7 +588,8 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     return fail(code, "qsm_create: %s", what);
   };
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(QSM_ERR_HIP, "stream");
-  // wavenumbers, quasi.cpp:397-431 (same expressions: pi m/(dx N), pi (m+1)/(dx N))
+  // wavenumbers, quasi.cpp:397-431 (same expressions: pi m/(dx N), pi (m+1)/(dx N)),
+  // then the three 1-D factors of the density filter exp(-filter q^2)
   std::vector<double> w;
   for (int a = 0; a < 3; ++a) {
     const int N = nn[a];
@@ -585,12 +597,30 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     for (int m = 0; m < N; ++m) w.push_back(1.0 * M_PI * m / (d * N));
     for (int m = 0; m < N; ++m) w.push_back(1.0 * M_PI * (m + 1) / (d * N));
   }
+  const size_t nwav = w.size();
+  for (int a = 0; a < 3; ++a) {
+    const int N = nn[a];
+    const double L = N * 2.0 * M_PI / (p.q0 * p.nw), d = L / N;
+    for (int m = 0; m < N; ++m) {
+      const long double q = 1.0 * M_PI * m / (d * N);
+      w.push_back((double)expl(-(long double)p.filter * q * q));
+    }
+  }
   if (hipMalloc(&h->wtab, w.size() * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "wavenumber tables");
   if (hipMemcpy(h->wtab, w.data(), w.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
     return bail(QSM_ERR_HIP, "wavenumber upload");
   h->waves.qx = h->wtab; h->waves.sx = h->wtab + p.nx;
   h->waves.qy = h->wtab + 2 * p.nx; h->waves.sy = h->waves.qy + p.ny;
   h->waves.qz = h->wtab + 2 * p.nx + 2 * p.ny; h->waves.sz = h->waves.qz + p.nz;
+  h->waves.fx = h->wtab + nwav; h->waves.fy = h->waves.fx + p.nx; h->waves.fz = h->waves.fy + p.ny;
+  // the filter store also scales psihat at setup with a unit filter
+  if (hipMalloc(&h->ones, (size_t)std::max(p.nx, std::max(p.ny, p.nz)) * 8) != hipSuccess)
+    return bail(QSM_ERR_NOMEM, "unit filter");
+  {
+    std::vector<double> one((size_t)std::max(p.nx, std::max(p.ny, p.nz)), 1.0);
+    if (hipMemcpy(h->ones, one.data(), one.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(QSM_ERR_HIP, "unit filter upload");
+  }
   for (int a = 0; a < 3; ++a) {
     h->tw[a] = twiddles(nn[a], h->fp32);
     if (!h->tw[a]) return bail(QSM_ERR_NOMEM, "twiddle table");
@@ -624,12 +654,12 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   for (int a = 0; a < 3; ++a) h->tw[a] = twiddles(nn[a], 0);
   int rc = run_pass(h, 2, false, false, plain_load(h->psihat_old), plain_store(h->psihat), true, nullptr);
   if (!rc) rc = run_pass(h, 1, false, false, plain_load(h->psihat), plain_store(h->psihat), false, nullptr);
-  // last pass: scale via the filter store with filter exponent 0 (exp(0) = 1)
+  // last pass: scale via the filter store with unit filter factors
   StoreOp S{}; S.code = ST_FILTER; S.d[0] = h->psihat;
-  const double filt = h->phys.filter;
-  h->phys.filter = 0.0;
+  const Waves wsaved = h->waves;
+  h->waves.fx = h->waves.fy = h->waves.fz = h->ones;
   if (!rc) rc = run_pass(h, 0, false, false, plain_load(h->psihat), S, false, nullptr);
-  h->phys.filter = filt;
+  h->waves = wsaved;
   for (int a = 0; a < 3; ++a) h->tw[a] = tw_saved[a];
   h->fp32 = saved_fp32;
   if (rc) return rc;
@@ -655,12 +685,59 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   return QSM_OK;
 }
 
+namespace {
+
+// Everything a step's kernel arguments depend on: the buffers behind the
+// ping-pong fields and the phase flags of step n = h->step + 1.
+std::vector<long long> step_key(const qsm_handle_s* h) {
+  const long long n = h->step + 1;
+  const qsm_params& p = h->p;
+  return {(long long)h->psi_cur, (long long)h->rho_cur, (long long)(uintptr_t)h->psihat,
+          n == 1, n >= 2, n > p.divv_switch, n > p.divv_switch + 1};
+}
+
+int one_step(qsm_handle_s* h) {
+  return h->p.variant == QSM_STB ? stb_step(h) : quasi_step(h);
+}
+
+// One step: replay its graph if this signature was captured before; else
+// capture it (the capture also performs the host-side bookkeeping), then
+// launch.  Timing (per-launch events) needs eager launches.
+int graph_step(qsm_handle_s* h) {
+  if (!h->use_graphs || h->timer.on) return one_step(h);
+  const auto key = step_key(h);
+  auto it = h->graphs.find(key);
+  if (it != h->graphs.end()) {
+    h->dry = true;
+    const int rc = one_step(h);  // bookkeeping only
+    h->dry = false;
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(it->second, h->stream));
+    return QSM_OK;
+  }
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = one_step(h);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "step capture: %s", hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  h->graphs[key] = ex;
+  HIPCHK(hipGraphLaunch(ex, h->stream));
+  return QSM_OK;
+}
+
+}  // namespace
+
 extern "C" int qsm_step(qsm_handle h, int nsteps) {
   if (!h) return fail(QSM_ERR_ARG, "qsm_step: null handle");
   if (!h->ready) return fail(QSM_ERR_STATE, "qsm_step: call qsm_set_psi first");
   if (nsteps < 0) return fail(QSM_ERR_ARG, "qsm_step: negative step count");
   for (int s = 0; s < nsteps; ++s) {
-    int rc = h->p.variant == QSM_STB ? stb_step(h) : quasi_step(h);
+    int rc = graph_step(h);
     if (rc) return rc;
   }
   if (h->p.variant == QSM_STB && nsteps > 0) {
@@ -668,6 +745,12 @@ extern "C" int qsm_step(qsm_handle h, int nsteps) {
     if (rc) return rc;
   }
   HIPCHK(hipStreamSynchronize(h->stream));
+  return QSM_OK;
+}
+
+extern "C" int qsm_set_graphs(qsm_handle h, int enable) {
+  if (!h) return fail(QSM_ERR_ARG, "qsm_set_graphs: null handle");
+  h->use_graphs = enable != 0;
   return QSM_OK;
 }
 

@@ -27,13 +27,25 @@ struct PassArgs {
   int pad_;
 };
 
+// A batched pass: `count` instances with device descriptors and the
+// exclusive prefix sum of their tile counts (count + 1 entries).
+struct BatchLaunch {
+  const PassArgs* descs;
+  const int* prefix;
+  int count;
+  long total_tiles;
+};
+
 // One pass of a 3D r2r transform along `axis` (0 = x, 1 = y, 2 = z) of length n.
 // inv = REDFT01/RODFT01 (DCT-III/DST-III); otherwise REDFT10/RODFT10.
 // ld / st: the LoadCode / StoreCode of the fused operators (A.ld.code and
 // A.st.code are ignored); combinations that are not compiled return
 // hipErrorInvalidValue.
+// B != nullptr launches the batched kernel over B's instances instead; A is
+// then ignored (every instance's arguments come from B->descs).
 hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
-                       long* nblocks, hipStream_t s);
+                       long* nblocks, hipStream_t s, const BatchLaunch* B = nullptr);
+long pass_tiles(int prec_fp32, int n, int axis, const Geo& g);
 bool pass_supported(int n);
 bool step_supported(int n);
 int pass_tile_k(int prec_fp32, int n, int axis);
@@ -44,5 +56,8 @@ hipError_t launch_diag(int prec_fp32, const void* psi, const void* pold, const v
                        const void* F, long n, const Phys& P, double* part, int nblocks,
                        double* out4, hipStream_t s);
 hipError_t launch_finish(const double* part, long nparts, double* out, hipStream_t s);
+// count independent compensated sums: out[q][0] = sum of parts[q][0 .. n[q])
+hipError_t launch_finish_batch(const double* const* parts, const long* n, int count,
+                               double* const* out, hipStream_t s);
 
 }  // namespace qsm

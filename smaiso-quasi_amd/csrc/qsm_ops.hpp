@@ -18,9 +18,12 @@ struct Geo {
   int pad_;
 };
 
-// FP64 wavenumber tables, quasi.cpp:397-431: q[m] = pi m/L, s[m] = pi (m+1)/L
+// FP64 wavenumber tables, quasi.cpp:397-431: q[m] = pi m/L, s[m] = pi (m+1)/L,
+// and the density filter exp(-filter K^2) = fx[i] fy[j] fz[k] with
+// f_c[m] = exp(-filter q_c[m]^2) (quasi.cpp:1261 is separable in K^2).
 struct Waves {
   const double *qx, *sx, *qy, *sy, *qz, *sz;
+  const double *fx, *fy, *fz;
 };
 
 // physical constants of the step (quasi.cpp:159-219, quasi-stb.cpp:149-220)
@@ -171,8 +174,8 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
     const long nyz = (long)G.ny * G.nz;
     const bool in = i < G.nx - 1;  // v_x = 0 at i = Nx-1
     const double nu = P.nu;
-    const double m2 = sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-    const double cm1 = P.scale / (nu * m2), cm2 = W.sx[i] / m2;
+    const double rm2 = 1.0 / (sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]));
+    const double cm1 = (P.scale / nu) * rm2, cm2 = W.sx[i] * rm2;
     const double sx = rd<T>(L.a[0], idx);
     const double syv = rd_if<T>(L.a[1], idx, nyz - G.nz, in && j > 0);
     const double szv = rd_if<T>(L.a[2], idx, nyz - 1, in && k > 0);
@@ -191,21 +194,21 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
     const long dx = (long)(ih - i) * nyz;
     double vv;
     if constexpr (LD == LD_VEL1) {
-      const double m2 = sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]);
+      const double rm2 = 1.0 / (sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]));
       const double sy = rd<T>(L.a[1], idx);
       const double sxa = rd_if<T>(L.a[0], idx, dx + G.nz, in && i > 0);
       const double szv = rd_if<T>(L.a[2], idx, G.nz - 1, in && k > 0);
       const double dot = W.qx[i] * sxa + W.sy[j] * sy + W.qz[k] * szv;
       const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, G.nz, in) : 0.0;
-      vv = (P.scale / (nu * m2)) * (sy - (W.sy[j] / m2) * dot + nu * W.sy[j] * dd);
+      vv = ((P.scale / nu) * rm2) * (sy - (W.sy[j] * rm2) * dot + nu * W.sy[j] * dd);
     } else {
-      const double m2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]);
+      const double rm2 = 1.0 / (sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]));
       const double sz = rd<T>(L.a[2], idx);
       const double sxb = rd_if<T>(L.a[0], idx, dx + 1, in && i > 0);
       const double syv = rd_if<T>(L.a[1], idx, 1 - (long)G.nz, in && j > 0);
       const double dot = W.qx[i] * sxb + W.qy[j] * syv + W.sz[k] * sz;
       const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, 1, in) : 0.0;
-      vv = (P.scale / (nu * m2)) * (sz - (W.sz[k] / m2) * dot + nu * W.sz[k] * dd);
+      vv = ((P.scale / nu) * rm2) * (sz - (W.sz[k] * rm2) * dot + nu * W.sz[k] * dd);
     }
     return (T)(in ? vv : 0.0);
   } else if constexpr (LD == LD_SCALE) {
@@ -249,42 +252,78 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
   }
 }
 
+// Store operators that read fields besides the pass output (the update reads
+// psihat, psihat_old, N_old) split into a load phase and an apply phase, so a
+// caller can issue the loads of many elements before any store: the compiler
+// cannot move a load above an earlier store to a possibly aliasing array.
+struct StIn {
+  double ph, pho, nold;
+};
+
+template <int ST>
+constexpr bool store_reads() {
+  return ST == ST_UPDATE || ST == ST_UPDATE_STB;
+}
+
+template <typename T, int ST>
+__device__ __forceinline__ StIn store_inputs(const StoreOp& S, long idx) {
+  StIn in{0.0, 0.0, 0.0};
+  if constexpr (store_reads<ST>()) {
+    in.ph = ((const double*)S.s[0])[idx];
+    in.pho = ((const double*)S.d[0])[idx];
+    in.nold = (double)((const T*)S.d[1])[idx];
+  }
+  return in;
+}
+
+// d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out);
+// s0: psihat (double, in).  quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
+template <typename T, int ST>
+__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Waves& W, int i,
+                                            int j, int k, long idx, T v, const StIn& in) {
+  const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+  const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
+  double nh = (double)v;
+  if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
+  double nold = in.nold;
+  if (S.first) nold = nh;
+  else if (S.dc_old && idx == 0) nold = S.dc_old[0];
+  double nw;
+  if constexpr (ST == ST_UPDATE) {
+    const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
+    nw = (c1 * in.ph - 0.5 * in.pho + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
+  } else {
+    const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
+    nw = (c1 * in.ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
+  }
+  ((double*)S.d[0])[idx] = nw;
+  ((T*)S.d[1])[idx] = (T)nh;
+  if (S.dc_out && idx == 0) S.dc_out[0] = nh;
+}
+
+// Load operators that read many fields or do FP64 projection arithmetic are
+// evaluated in chunks of 8 elements (register pressure); the rest in one go.
+template <int LD>
+constexpr bool load_heavy() {
+  return LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_PRESSURE ||
+         LD == LD_DIVV || LD == LD_NONLIN || LD == LD_NONLIN_P || LD == LD_STRESS ||
+         LD == LD_RHOSTAR || LD == LD_STRESS_STB || LD == LD_NONLIN_STB;
+}
+
 template <typename T, int ST>
 __device__ __forceinline__ void store_eval(const StoreOp& S, const Phys& P, const Waves& W, int i,
                                            int j, int k, long idx, T v) {
   if constexpr (ST == ST_PLAIN) {
     ((T*)S.d[0])[idx] = v;
   } else if constexpr (ST == ST_FILTER) {
-    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
     double raw = (double)v;
     if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
-    const double r = P.scale * raw * exp(-P.filter * mq2);
+    const double r = P.scale * raw * (W.fx[i] * W.fy[j] * W.fz[k]);
     ((T*)S.d[0])[idx] = (T)r;
     if (S.dc_out && idx == 0) S.dc_out[0] = r;
   } else if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
-    // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out)
-    // s0: psihat (double, in)
-    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-    const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-    double nh = (double)v;
-    if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
-    double nold;
-    if (S.first) nold = nh;
-    else if (S.dc_old && idx == 0) nold = S.dc_old[0];
-    else nold = (double)((const T*)S.d[1])[idx];
-    const double ph = ((const double*)S.s[0])[idx];
-    double* pho = (double*)S.d[0];
-    double nw;
-    if constexpr (ST == ST_UPDATE) {
-      const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
-      nw = (c1 * ph - 0.5 * pho[idx] + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
-    } else {
-      const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
-      nw = (c1 * ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
-    }
-    pho[idx] = nw;
-    ((T*)S.d[1])[idx] = (T)nh;
-    if (S.dc_out && idx == 0) S.dc_out[0] = nh;
+    StIn in = store_inputs<T, ST>(S, idx);
+    store_apply<T, ST>(S, P, W, i, j, k, idx, v, in);
   } else {
     static_assert(ST == ST_PLAIN, "unknown store operator");
   }

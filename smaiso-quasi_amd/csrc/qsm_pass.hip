@@ -8,7 +8,7 @@ namespace qsm {
 
 #define QSM_DECL(T, AX) \
   hipError_t launch_pass_##T##_##AX(int n, bool inv, int ld, int st, const PassArgs& A, long* nb, \
-                                    hipStream_t s);
+                                    hipStream_t s, const BatchLaunch* B);
 QSM_DECL(float, 0)
 QSM_DECL(float, 1)
 QSM_DECL(float, 2)
@@ -18,16 +18,18 @@ QSM_DECL(double, 2)
 #undef QSM_DECL
 
 hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
-                       long* nb, hipStream_t s) {
+                       long* nb, hipStream_t s, const BatchLaunch* B) {
   if (prec_fp32) {
-    if (axis == 0) return launch_pass_float_0(n, inv, ld, st, A, nb, s);
-    if (axis == 1) return launch_pass_float_1(n, inv, ld, st, A, nb, s);
-    return launch_pass_float_2(n, inv, ld, st, A, nb, s);
+    if (axis == 0) return launch_pass_float_0(n, inv, ld, st, A, nb, s, B);
+    if (axis == 1) return launch_pass_float_1(n, inv, ld, st, A, nb, s, B);
+    return launch_pass_float_2(n, inv, ld, st, A, nb, s, B);
   }
-  if (axis == 0) return launch_pass_double_0(n, inv, ld, st, A, nb, s);
-  if (axis == 1) return launch_pass_double_1(n, inv, ld, st, A, nb, s);
-  return launch_pass_double_2(n, inv, ld, st, A, nb, s);
+  if (axis == 0) return launch_pass_double_0(n, inv, ld, st, A, nb, s, B);
+  if (axis == 1) return launch_pass_double_1(n, inv, ld, st, A, nb, s, B);
+  return launch_pass_double_2(n, inv, ld, st, A, nb, s, B);
 }
+
+long pass_tiles(int prec_fp32, int n, int axis, const Geo& g) { return pass_blocks(prec_fp32, n, axis, g); }
 
 bool step_supported(int n) { return QSM_IS_STEP_N(n); }
 
@@ -139,6 +141,31 @@ hipError_t launch_diag(int prec_fp32, const void* psi, const void* pold, const v
     hipLaunchKernelGGL(k_diag<double>, dim3(nblocks), dim3(256), 0, s, (const double*)psi,
                        (const double*)pold, (const double*)rho, (const double*)F, n, P, part);
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, (long)nblocks, 4, 4, out4);
+  return hipGetLastError();
+}
+
+// One block per sum: out[q][0] = compensated sum of parts[q][0 .. n[q]).
+__global__ void __launch_bounds__(256)
+    k_finish_batch(const double* const* __restrict__ parts, const long* __restrict__ n,
+                   double* const* __restrict__ out) {
+  __shared__ KSum red[256];
+  const int q = blockIdx.x;
+  const double* p = parts[q];
+  KSum s{0, 0};
+  for (long b = threadIdx.x; b < n[q]; b += blockDim.x) s.add(p[b]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x].merge(red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q][0] = red[0].s + red[0].c;
+}
+
+hipError_t launch_finish_batch(const double* const* parts, const long* n, int count,
+                               double* const* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finish_batch, dim3(count), dim3(256), 0, s, parts, n, out);
   return hipGetLastError();
 }
 

@@ -92,9 +92,9 @@ struct TileGeo {
   }
 };
 
+// One workgroup's work: tile `tile` of the pass described by A.
 template <typename T, int N, int AX, bool INV, int LD, int ST>
-__global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
-    k_pass(const PassArgs A) {
+__device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   using C = TileCfg<T, N, AX>;
   using PL = Plan<N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
@@ -115,11 +115,11 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
   tg.nx = G.nx; tg.ny = G.ny; tg.nz = G.nz;
   tg.nrows = G.nx * G.ny;
   if constexpr (AX == kAxZ) {
-    tg.row0 = (int)blockIdx.x * TK;
+    tg.row0 = tile * TK;
   } else {
     const int tiles_k = (G.nz + TK - 1) / TK;
-    const int a = blockIdx.x / tiles_k;
-    tg.k0 = (blockIdx.x % tiles_k) * TK;
+    const int a = tile / tiles_k;
+    tg.k0 = (tile % tiles_k) * TK;
     if constexpr (AX == kAxY) tg.t_i = a; else tg.t_j = a;
   }
 
@@ -130,29 +130,33 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
 
   if constexpr (!INV) {
     if constexpr (AX == kAxZ) {
-      // coalesced row loads into LDS in Makhoul order; all loads are issued
-      // before any value is used
+      // coalesced row loads into LDS in Makhoul order; the loads of a chunk
+      // are all issued before any of its values is used
       constexpr int PER = TK * N / BLOCK;
-      T xs[PER];
+      constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+#pragma unroll 1
+      for (int s0 = 0; s0 < PER; s0 += CH) {
+        T xs[CH];
 #pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        int i, j, k;
-        long idx;
-        tg.at(e / N, e % N, i, j, k, idx);
-        xs[s] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
-      }
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          int i, j, k;
+          long idx;
+          tg.at(e / N, e % N, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
+        }
 #pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        const int q = e / N, n = e % N;
-        int i, j, k;
-        long idx;
-        T x = tg.at(q, n, i, j, k, idx) ? xs[s] : (T)0;
-        if (A.part) dsum += (double)x;
-        if (dst && (n & 1)) x = -x;
-        const int vq = vpos<N>(n);
-        rbuf[((vq >> 1) * TKP + q) * 2 + (vq & 1)] = x;
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          const int q = e / N, n = e % N;
+          int i, j, k;
+          long idx;
+          T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
+          if (A.part) dsum += (double)x;
+          if (dst && (n & 1)) x = -x;
+          const int vq = vpos<N>(n);
+          rbuf[((vq >> 1) * TKP + q) * 2 + (vq & 1)] = x;
+        }
       }
       __syncthreads();
       gather<T, M, E, TKP, R1>(v, buf, p, t);
@@ -211,9 +215,13 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
       const cx<T> Wm = cmul(twq<T, N>(tq, M - kk), conj(Ev - wO));
       y0[b] = Wk.x; y1[b] = -Wk.y; y2[b] = Wm.x; y3[b] = -Wm.y;
     }
+    // z passes and store operators that read other fields write Y through
+    // LDS and store it row-coalesced in chunks; plain strided stores go
+    // straight from registers.
+    constexpr bool STAGED = (AX == kAxZ) || store_reads<ST>();
     auto put = [&](int o, T val) {
       const int n = dst ? (N - 1 - o) : o;
-      if constexpr (AX == kAxZ) {
+      if constexpr (STAGED) {
         rbuf[((n >> 1) * TKP + p) * 2 + (n & 1)] = val;
       } else {
         int i, j, k;
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
         if (tg.at(p, n, i, j, k, idx)) store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, val);
       }
     };
-    if constexpr (AX == kAxZ) __syncthreads();
+    if constexpr (STAGED) __syncthreads();
 #pragma unroll
     for (int b = 0; b <= NPRE; ++b) {
       int kk = t + b * TP;
@@ -234,45 +242,67 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
         if (kk > 0) put(M + kk, y3[b]);
       }
     }
-    if constexpr (AX == kAxZ) {
+    if constexpr (STAGED) {
       __syncthreads();
       constexpr int PER = TK * N / BLOCK;
+      constexpr int CH = store_reads<ST>() ? (PER < 8 ? PER : 8) : PER;
+#pragma unroll 1
+      for (int s0 = 0; s0 < PER; s0 += CH) {
+        StIn in[CH];
 #pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        const int q = e / N, n = e % N;
-        int i, j, k;
-        long idx;
-        if (!tg.at(q, n, i, j, k, idx)) continue;
-        store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, rbuf[((n >> 1) * TKP + q) * 2 + (n & 1)]);
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          int i, j, k;
+          long idx;
+          tg.at(q, n, i, j, k, idx);
+          in[c] = store_inputs<T, ST>(A.st, idx);
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          int i, j, k;
+          long idx;
+          if (!tg.at(q, n, i, j, k, idx)) continue;
+          const T y = rbuf[((n >> 1) * TKP + q) * 2 + (n & 1)];
+          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y, in[c]);
+          else store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y);
+        }
       }
     }
   } else {
     // ---- load X in natural order (DST-III: reversed) into LDS
     {
       constexpr int PER = TK * N / BLOCK;
-      T xs[PER];
+      constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+#pragma unroll 1
+      for (int s0 = 0; s0 < PER; s0 += CH) {
+        T xs[CH];
 #pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        int q, n;
-        if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
-        int i, j, k;
-        long idx;
-        tg.at(q, n, i, j, k, idx);
-        xs[s] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
-      }
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          int i, j, k;
+          long idx;
+          tg.at(q, n, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
+        }
 #pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        int q, n;
-        if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
-        int i, j, k;
-        long idx;
-        const T x = tg.at(q, n, i, j, k, idx) ? xs[s] : (T)0;
-        if (A.part) dsum += (double)x;
-        const int pos = dst ? (N - 1 - n) : n;
-        rbuf[((pos >> 1) * TKP + q) * 2 + (pos & 1)] = x;
+        for (int c = 0; c < CH; ++c) {
+          const int e = tid + (s0 + c) * BLOCK;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          int i, j, k;
+          long idx;
+          const T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
+          if (A.part) dsum += (double)x;
+          const int pos = dst ? (N - 1 - n) : n;
+          rbuf[((pos >> 1) * TKP + q) * 2 + (pos & 1)] = x;
+        }
       }
     }
     __syncthreads();
@@ -348,21 +378,51 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < (BLOCK + 63) / 64; ++w) s += red[w];
-      A.part[blockIdx.x] = s;
+      A.part[tile] = s;
     }
   }
 }
 
-
 template <typename T, int N, int AX, bool INV, int LD, int ST>
-hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s) {
-  using C = TileCfg<T, N, AX>;
-  long nb;
-  if (AX == kAxZ) nb = ((long)A.geo.nx * A.geo.ny + C::TK - 1) / C::TK;
-  else {
-    const long tiles_k = (A.geo.nz + C::TK - 1) / C::TK;
-    nb = (AX == kAxY ? (long)A.geo.nx : (long)A.geo.ny) * tiles_k;
+__global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK)) k_pass(const PassArgs A) {
+  pass_tile<T, N, AX, INV, LD, ST>(A, (int)blockIdx.x);
+}
+
+// Batched pass over the instances of an ensemble that share this pass's
+// length N: workgroup b runs tile b - prefix[q] of instance q, where
+// prefix[q] <= b < prefix[q+1] (prefix[0] = 0, prefix[count] = grid size).
+template <typename T, int N, int AX, bool INV, int LD, int ST>
+__global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
+    k_pass_batch(const PassArgs* __restrict__ descs, const int* __restrict__ prefix, int count) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = count;  // invariant: prefix[lo] <= b < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid;
   }
+  pass_tile<T, N, AX, INV, LD, ST>(descs[lo], b - prefix[lo]);
+}
+
+
+template <typename T, int N, int AX>
+long tiles_of(const Geo& g) {
+  using C = TileCfg<T, N, AX>;
+  if (AX == kAxZ) return ((long)g.nx * g.ny + C::TK - 1) / C::TK;
+  const long tiles_k = (g.nz + C::TK - 1) / C::TK;
+  return (AX == kAxY ? (long)g.nx : (long)g.ny) * tiles_k;
+}
+
+// Single pass (B == nullptr) or batched pass (B: device descriptors).
+template <typename T, int N, int AX, bool INV, int LD, int ST>
+hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const BatchLaunch* B) {
+  using C = TileCfg<T, N, AX>;
+  if (B) {
+    if (B->total_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_pass_batch<T, N, AX, INV, LD, ST>), dim3((unsigned)B->total_tiles),
+                       dim3(C::BLOCK), 0, s, B->descs, B->prefix, B->count);
+    return hipGetLastError();
+  }
+  const long nb = tiles_of<T, N, AX>(A.geo);
   if (nblocks_out) *nblocks_out = nb;
   if (nb <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_pass<T, N, AX, INV, LD, ST>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
@@ -375,14 +435,15 @@ hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s) {
 // forward z passes load the real-space operators, forward x passes store the
 // spectral epilogues, inverse x passes load the spectral prologues.
 template <typename T, int AX, int N, bool STEP>
-hipError_t dispatch_ops(bool inv, int ld, int st, const PassArgs& A, long* nb, hipStream_t s) {
+hipError_t dispatch_ops(bool inv, int ld, int st, const PassArgs& A, long* nb, hipStream_t s,
+                        const BatchLaunch* B) {
   if (ld == LD_PLAIN && st == ST_PLAIN)
-    return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN>(A, nb, s)
-               : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN>(A, nb, s);
+    return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN>(A, nb, s, B)
+               : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN>(A, nb, s, B);
   if constexpr (STEP && AX == kAxZ) {
     if (!inv && st == ST_PLAIN) {
       switch (ld) {
-#define QSM_L(OP) case OP: return launch_one<T, N, AX, false, OP, ST_PLAIN>(A, nb, s);
+#define QSM_L(OP) case OP: return launch_one<T, N, AX, false, OP, ST_PLAIN>(A, nb, s, B);
         QSM_L(LD_ZERO) QSM_L(LD_RHOSTAR) QSM_L(LD_DIVV) QSM_L(LD_STRESS) QSM_L(LD_NONLIN)
         QSM_L(LD_NONLIN_P) QSM_L(LD_STRESS_STB) QSM_L(LD_NONLIN_STB) QSM_L(LD_NONLIN_STB0)
 #undef QSM_L
@@ -393,7 +454,7 @@ hipError_t dispatch_ops(bool inv, int ld, int st, const PassArgs& A, long* nb, h
   if constexpr (STEP && AX == kAxX) {
     if (!inv && ld == LD_PLAIN) {
       switch (st) {
-#define QSM_S(OP) case OP: return launch_one<T, N, AX, false, LD_PLAIN, OP>(A, nb, s);
+#define QSM_S(OP) case OP: return launch_one<T, N, AX, false, LD_PLAIN, OP>(A, nb, s, B);
         QSM_S(ST_FILTER) QSM_S(ST_UPDATE) QSM_S(ST_UPDATE_STB)
 #undef QSM_S
         default: break;
@@ -401,7 +462,7 @@ hipError_t dispatch_ops(bool inv, int ld, int st, const PassArgs& A, long* nb, h
     }
     if (inv && st == ST_PLAIN) {
       switch (ld) {
-#define QSM_L(OP) case OP: return launch_one<T, N, AX, true, OP, ST_PLAIN>(A, nb, s);
+#define QSM_L(OP) case OP: return launch_one<T, N, AX, true, OP, ST_PLAIN>(A, nb, s, B);
         QSM_L(LD_PLAIN_D) QSM_L(LD_COEF_F) QSM_L(LD_COEF_G) QSM_L(LD_GRAD) QSM_L(LD_GRAD_D)
         QSM_L(LD_LAPGRAD) QSM_L(LD_SH) QSM_L(LD_PRESSURE) QSM_L(LD_VEL0) QSM_L(LD_VEL1)
         QSM_L(LD_VEL2) QSM_L(LD_SCALE)

@@ -76,6 +76,7 @@ SIGNATURES = {
     "qsm_create": ([ctypes.POINTER(Params), _IP, ctypes.POINTER(_VP)], _IP),
     "qsm_set_psi": ([_VP, _DP], _IP),
     "qsm_step": ([_VP, _IP], _IP),
+    "qsm_set_graphs": ([_VP, _IP], _IP),
     "qsm_run_l1": ([_VP, _IP, _IP, ctypes.c_double, ctypes.POINTER(_IP)], _IP),
     "qsm_get_field": ([_VP, _IP, _DP], _IP),
     "qsm_get_diag": ([_VP, ctypes.POINTER(Diag)], _IP),
@@ -183,6 +184,9 @@ class Sim:
 
     def step(self, n=1):
         _check(lib().qsm_step(self.h, n))
+
+    def graphs(self, enable=True):
+        _check(lib().qsm_set_graphs(self.h, 1 if enable else 0))
 
     def run_l1(self, max_steps, step_l1=50, lim_l1=1e-6):
         done = ctypes.c_int()

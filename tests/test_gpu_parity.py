@@ -80,7 +80,7 @@ STEP_FIELDS = ["F", "G", "GX", "GY", "GZ", "RX", "RY", "RZ", "LX", "LY", "LZ", "
                "PSIHAT_OLD", "PSI", "PSI_OLD"]
 
 
-def _cmp_fields(sim, orc, names, tol, tag):
+def _cmp_fields(sim, orc, names, tol, tag, tols=None):
     for f in names:
         a = sim.field(f)
         b = orc.field(f)
@@ -92,8 +92,9 @@ def _cmp_fields(sim, orc, names, tol, tag):
             assert not np.any(a), (tag, f)
             continue
         e = rel_l2(a, b)
-        print(f"{tag} {f}: rel_l2 {e:.2e}", flush=True)
-        assert e < tol, (tag, f, e)
+        t = max(tol, (tols or {}).get(f, 0.0))
+        print(f"{tag} {f}: rel_l2 {e:.2e} (tol {t:.1e})", flush=True)
+        assert e < t, (tag, f, e)
 
 
 def test_setup_matches(qsm_mod, oracle_mod):
@@ -101,14 +102,44 @@ def test_setup_matches(qsm_mod, oracle_mod):
     _cmp_fields(sim, orc, ["PSIHAT", "PSI"], 1e-13, "setup")
 
 
+def _oracle_sensitivity(oracle_mod, d, psi, steps, names):
+    """Relative change of each field when psi is perturbed by 1e-16 (relative):
+    the round-off floor of a field in the oracle itself."""
+    out = []
+    runs = []
+    for pert in (0.0, 1e-16):
+        st = oracle_mod.State(oracle_mod.make_params(**d))
+        p = psi * (1 + pert * np.random.default_rng(9).standard_normal(psi.shape))
+        st.set_psi(p)
+        fields = []
+        for _ in range(steps):
+            st.step(1)
+            fields.append({f: st.field(f) for f in names})
+        runs.append(fields)
+    for s in range(steps):
+        out.append({f: rel_l2(runs[1][s][f], runs[0][s][f]) for f in names})
+    return out
+
+
 @pytest.mark.parametrize("compat", (0, 4))
 def test_one_step_all_fields_fp64(qsm_mod, oracle_mod, compat):
-    # divv_switch = 1 makes the divergence branch live at step 2 and the pressure term at step 3
-    sim, orc = _pair(qsm_mod, oracle_mod, shape=(32, 48, 64), compat_ranks=compat, divv_switch=1)
+    """Every intermediate field, three steps, divergence live from step 2 and
+    the pressure term from step 3 (divv_switch = 1).  Once D is live, v is the
+    small solenoidal remainder of the stress and amplifies round-off by ~1e6
+    (a 1e-16 perturbation of psi moves it by ~1e-10 in the oracle itself), so
+    each field's tolerance is max(1e-10, 10 x the oracle's own sensitivity)."""
+    import ics
+    shape = (32, 48, 64)
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=shape, compat_ranks=compat, divv_switch=1)
+    d = ics.params_dict("interface64", compat_ranks=compat, divv_switch=1)
+    d["nx"], d["ny"], d["nz"] = shape
+    sens = _oracle_sensitivity(oracle_mod, d, ics.workload_psi("interface64", shape=shape), 3,
+                               ["VX", "VY", "VZ", "P", "DIVV", "DHAT"])
     for s in range(3):
         sim.step(1)
         orc.step(1)
-        _cmp_fields(sim, orc, STEP_FIELDS, 1e-10, f"compat{compat} step{s + 1}")
+        tols = {f: 10 * v for f, v in sens[s].items()}
+        _cmp_fields(sim, orc, STEP_FIELDS, 1e-10, f"compat{compat} step{s + 1}", tols)
 
 
 def test_interface64_fp64_100_steps(qsm_mod, oracle_mod):
@@ -154,3 +185,26 @@ def test_mixed_vs_oracle(qsm_mod, oracle_mod):
         assert e < 1e-5, (f, e)
     dg, do = sim.diag(), orc.diag()
     assert abs(dg["mass"] - do["mass"]) <= 1e-7 * abs(do["mass"])
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_graph_replay_matches_eager(qsm_mod, prec):
+    """Steps replayed from captured HIP graphs equal eager launches bit for
+    bit, across several replays of each captured signature with eager work
+    (field reads, diagnostics) and synchronisations between them."""
+    import ics
+    d = ics.params_dict("interface64", divv_switch=3)
+    d["nx"], d["ny"], d["nz"] = (32, 48, 64)
+    qp = qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_})
+    psi = ics.workload_psi("interface64", shape=(32, 48, 64))
+    g = qsm_mod.Sim(qp, prec)
+    e = qsm_mod.Sim(qp, prec)
+    e.graphs(False)
+    g.set_psi(psi)
+    e.set_psi(psi)
+    for chunk in (1, 1, 2, 3, 1, 4):
+        g.step(chunk)
+        e.step(chunk)
+        for f in ("PSI", "RHO", "VX", "PSIHAT", "NHAT"):
+            assert np.array_equal(g.field(f), e.field(f)), (chunk, f)
+        assert g.diag() == e.diag()
