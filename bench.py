@@ -35,11 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # every pointwise stage fused into a transform pass): 194 field sweeps
 SWEEPS_PER_STEP = 194
 
-PASS_NAMES = {1: "inverse x pass", 2: "inverse y pass", 3: "inverse z pass",
-              4: "forward x pass", 5: "forward y pass", 6: "forward z pass"}
-PASS_SYMBOLS = {1: "k_pass<T, N, 0, true>", 2: "k_pass<T, N, 1, true>",
-                3: "k_pass<T, N, 2, true>", 4: "k_pass<T, N, 0, false>",
-                5: "k_pass<T, N, 1, false>", 6: "k_pass<T, N, 2, false>"}
+LOAD_OPS = ["plain", "plain_d", "zero", "coef_f", "coef_g", "grad", "grad_d", "lapgrad", "sh",
+            "rhostar", "divv", "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin",
+            "nonlin_p", "stress_stb", "nonlin_stb", "nonlin_stb0"]
+STORE_OPS = ["plain", "filter", "update", "update_stb"]
 
 
 def log(msg):
@@ -58,6 +57,35 @@ def time_steps(sim, steps):
     t0 = time.perf_counter()
     sim.step(steps)  # returns after a stream synchronisation
     return time.perf_counter() - t0
+
+
+def ensemble_run(qsm, ics, steps):
+    """Stability ensemble: 96 quasi-stb instances, grid edges from
+    {32, 48, 64, 96, 128} per axis, perturbation wavenumbers U(0.05, 1),
+    `steps` steps in FP64 and in mixed precision, each as batched kernels
+    replayed from one HIP graph per step signature."""
+    members = ics.ensemble_members(96, seed=4)
+    params = [qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_}) for d, _ in members]
+    pts = float(sum(d["nx"] * d["ny"] * d["nz"] for d, _ in members))
+    res = {"workload": "96 stb instances, edges {32,48,64,96,128}, Qi ~ U(0.05,1), amplitude 1e-3",
+           "steps": steps, "grid_points": pts}
+    growth = {}
+    for prec, tag in ((qsm.FP64, "fp64"), (qsm.MIXED, "mixed")):
+        e = qsm.Ensemble(params, prec)
+        e.autotune()
+        for i, (_, psi) in enumerate(members):
+            e.set_psi(i, psi)
+        e.step(2)  # first two steps capture the step signatures
+        t0 = time.perf_counter()
+        e.step(steps - 2)
+        dt = time.perf_counter() - t0
+        growth[tag] = e.growth()
+        e.close()
+        res[tag] = {"updates_per_s": pts * (steps - 2) / dt, "ms_per_step": dt / (steps - 2) * 1e3}
+    gf, gm = growth["fp64"], growth["mixed"]
+    res["growth_fp64_first8"] = [float(v) for v in gf[:8]]
+    res["growth_mixed_vs_fp64_max_rel"] = float(np.max(np.abs(gm - gf)) / np.max(np.abs(gf)))
+    return res
 
 
 def cpu_baseline(target_s=15.0):
@@ -90,6 +118,8 @@ def main():
     ap.add_argument("--size", type=int, default=512, help="cubic grid edge (default 512)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -105,17 +135,21 @@ def main():
 
     res = {}
     fields = {}
+    top = None
     for prec, tag in ((qsm.MIXED, "mixed"), (qsm.FP64, "fp64")):
         if prec == qsm.FP64 and args.no_fp64:
             continue
         log(f"{tag}: create + setup")
         sim = make_sim(qsm, ics, name, prec, shape)
+        sim.autotune()
         sim.set_psi(psi0)
         sim.step(args.warmup)
         sim.timing(True)
         log(f"{tag}: timing {args.steps} steps")
         dt = time_steps(sim, args.steps)
-        kt = {q: sim.timing_get(q) for q in range(1, 7)}
+        kt = sim.timing_get(0)
+        if prec == qsm.MIXED:
+            top = sim.timing_kernel(0)
         sim.timing(False)
         fields[tag] = sim.field("PSI")
         diag = sim.diag()
@@ -126,9 +160,11 @@ def main():
     pts = float(np.prod(shape))
     m = res["mixed"]
     value = pts * args.steps / m["s"]
-    dom = max(m["kt"], key=lambda q: m["kt"][q]["ms"])
-    k = m["kt"][dom]
+    k = top
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+    symbol = (f"void qsm::k_pass<float, {shape[0]}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+              f"{k['ld']}, {k['st']}>(qsm::PassArgs)")
+    direction = "inverse" if k["inv"] else "forward"
     out = {
         "metric": "grid-pt updates/s at 512^3, mixed vs FP64 on 1 GPU; % HBM roofline; psi rel-L2",
         "value": value,
@@ -147,9 +183,13 @@ def main():
                    "grid": list(shape), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"{PASS_NAMES[dom]} ({PASS_SYMBOLS[dom]}, T=float, N={shape[0]})",
+                     "kernel": symbol,
+                     "pass": f"{direction} {'xyz'[k['axis']]} pass, load op {LOAD_OPS[k['ld']]}, "
+                             f"store op {STORE_OPS[k['st']]}",
                      "avg_launch_ms": k["ms"] / max(1, k["launches"]),
-                     "launches": k["launches"]},
+                     "launches": k["launches"],
+                     "bytes_per_launch": k["bytes"] / max(1, k["launches"])},
+        "passes_GBs": m["kt"]["bytes"] / (m["kt"]["ms"] * 1e-3) / 1e9 if m["kt"]["ms"] > 0 else 0.0,
         "step_roofline": {"bytes_per_update": SWEEPS_PER_STEP * 4,
                           "achieved_GBs": value * SWEEPS_PER_STEP * 4 / 1e9,
                           "frac": value * SWEEPS_PER_STEP * 4 / 1e9 / HBM_PEAK_GBS},
@@ -163,6 +203,9 @@ def main():
         out["mixed_over_fp64"] = value / v64
         out["psi_rel_l2_mixed_vs_fp64"] = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
         out["steps_compared"] = args.warmup + args.steps
+    if not args.no_ensemble:
+        log("ensemble")
+        out["ensemble"] = ensemble_run(qsm, ics, args.ens_steps)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

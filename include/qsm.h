@@ -154,6 +154,40 @@ int qsm_get_field(qsm_handle h, int which, double* host);
 int qsm_get_diag(qsm_handle h, qsm_diag* d);
 int qsm_destroy(qsm_handle h);
 
+/* Full-state binary checkpoint (every buffer the step reads, the step count
+ * and the parameters) for a bit-exact resume.  Replaces the reference's
+ * psi<rank>.dat restart (quasi.cpp:692-717, 2051-2062), which keeps psi only
+ * at 6 digits.  Reading requires a handle created with the same parameters
+ * and precision. */
+int qsm_checkpoint_write(qsm_handle h, const char* path);
+int qsm_checkpoint_read(qsm_handle h, const char* path);
+
+/* ---- ensembles: independent instances (e.g. the stability runs of
+ * quasi-stb.cpp at different grid sizes) advanced together.  Each launch of
+ * the step runs as one batched kernel per distinct pencil length over all
+ * instances, and a whole step is replayed as one captured HIP graph.
+ * params: count parameter sets (same variant, divv_switch and hydro). */
+typedef struct qsm_ens_s* qsm_ens;
+int qsm_ens_create(const qsm_params* params, int count, int prec, qsm_ens* out);
+int qsm_ens_set_psi(qsm_ens e, int index, const double* psi);
+int qsm_ens_step(qsm_ens e, int nsteps);
+/* growth rate of every instance after the last step (quasi-stb.cpp:2147-2158) */
+int qsm_ens_growth(qsm_ens e, double* out);
+int qsm_ens_get_field(qsm_ens e, int index, int which, double* host);
+int qsm_ens_get_diag(qsm_ens e, int index, qsm_diag* d);
+int qsm_ens_set_graphs(qsm_ens e, int enable);
+int qsm_ens_destroy(qsm_ens e);
+
+/* ---- autotuner: times the tile-width variants of the plain transform passes
+ * on the handle's (or ensemble's) own grid(s), using a scratch field, and
+ * keeps the faster per axis and direction (and pencil length for ensembles).
+ * cache_path (may be NULL): JSON file of earlier choices, read and updated. */
+int qsm_autotune(qsm_handle h, const char* cache_path);
+/* set the variant (0 default width, 1 half width) of the plain passes along
+ * axis (0 x, 1 y, 2 z) in direction inv (1: DCT-III/DST-III) by hand */
+int qsm_set_variant(qsm_handle h, int axis, int inv, int variant);
+int qsm_ens_autotune(qsm_ens e, const char* cache_path);
+
 /* ---- measurement support (no counterpart in the reference, which only
  * prints total wall time at quasi.cpp:2292-2305): device time of the pass
  * kernels, measured with HIP events recorded around every launch on the
@@ -164,6 +198,11 @@ int qsm_destroy(qsm_handle h);
 int qsm_timing_enable(qsm_handle h, int enable);
 int qsm_timing_get(qsm_handle h, int which, double* total_ms, long long* launches,
                    double* alg_bytes);
+/* the rank-th kernel (0 = most device time) among those timed: the pass
+ * kernel k_pass<T, N, axis, inv, ld, st> with its load and store operator
+ * codes, its total time, launches and algorithmic bytes */
+int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, int* ld, int* st,
+                      double* total_ms, long long* launches, double* alg_bytes);
 
 #ifdef __cplusplus
 }

@@ -86,40 +86,43 @@ bool is_kind(int k) {
 bool kind_inv(int k) { return k == QSM_REDFT01 || k == QSM_RODFT01; }
 bool kind_dst(int k) { return k == QSM_RODFT01 || k == QSM_RODFT10; }
 
-// ---- per-launch timing with HIP events (pass categories: axis x direction)
+// ---- per-launch timing with HIP events.  Categories: one per kernel symbol
+// k_pass<T, N, axis, inv, LD, ST> (key = ((axis*2 + inv)*64 + LD)*16 + ST).
 struct Timer {
   bool on = false;
   struct Rec { hipEvent_t a, b; int cat; double bytes; };
+  struct Acc { double ms = 0, bytes = 0; long long count = 0; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
-  double ms[8] = {0};
-  double bytes[8] = {0};
-  long long count[8] = {0};
+  std::map<int, Acc> acc;
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
     hipEvent_t e;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
   }
   void drain() {
     for (auto& r : recs) {
       float t = 0;
-      hipEventSynchronize(r.b);
-      hipEventElapsedTime(&t, r.a, r.b);
-      ms[r.cat] += t; bytes[r.cat] += r.bytes; count[r.cat] += 1;
+      (void)hipEventSynchronize(r.b);
+      (void)hipEventElapsedTime(&t, r.a, r.b);
+      Acc& a = acc[r.cat];
+      a.ms += t; a.bytes += r.bytes; a.count += 1;
       pool.push_back(r.a); pool.push_back(r.b);
     }
     recs.clear();
   }
   void reset() {
     drain();
-    for (int c = 0; c < 8; ++c) { ms[c] = 0; bytes[c] = 0; count[c] = 0; }
+    acc.clear();
   }
   ~Timer() {
-    for (auto& r : recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-    for (auto e : pool) hipEventDestroy(e);
+    for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
+
+int timer_key(int axis, bool inv, int ld, int st) { return ((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st; }
 
 }  // namespace
 
@@ -229,6 +232,17 @@ enum Dsc {
 
 }  // namespace
 
+// One launch of a step, recorded instead of launched (ensembles batch the
+// same launch of many instances into one kernel).
+struct PassRec {
+  int kind;  // 0: pass, 1: FP64 finish of per-block partial sums
+  int axis, n, inv, ld, st, variant;
+  PassArgs A;
+  const double* part;
+  long nparts;
+  double* out;
+};
+
 struct qsm_handle_s {
   qsm_params p{};
   int prec = 0, fp32 = 0;
@@ -255,6 +269,8 @@ struct qsm_handle_s {
   // arguments depend on (ping-pong buffer state and phase flags)
   bool use_graphs = true;
   bool dry = false;  // sequence the step without launching (state bookkeeping only)
+  std::vector<PassRec>* rec = nullptr;  // when set, launches are recorded here
+  int variant[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // tile-width variant of plain passes [axis][inv]
   std::map<std::vector<long long>, hipGraphExec_t> graphs;
 };
 
@@ -318,14 +334,22 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   A.dst = dst ? 1 : 0;
   const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
   if (h->dry) return QSM_OK;
+  const int var = (L.code == LD_PLAIN && S.code == ST_PLAIN) ? h->variant[axis][inv ? 1 : 0] : 0;
+  if (h->rec) {
+    PassRec r{};
+    r.kind = 0; r.axis = axis; r.n = n; r.inv = inv; r.ld = L.code; r.st = S.code; r.A = A;
+    r.variant = var;
+    h->rec->push_back(r);
+    return QSM_OK;
+  }
   Timer& T = h->timer;
   hipEvent_t ea = nullptr, eb = nullptr;
   if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
-  hipError_t e = launch_pass(h->fp32, n, axis, inv, L.code, S.code, A, nullptr, h->stream);
+  hipError_t e = launch_pass(h->fp32, n, axis, inv, L.code, S.code, A, nullptr, h->stream, nullptr, var);
   if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch (axis %d, n %d): %s", axis, n, hipGetErrorString(e));
   if (T.on) {
     hipEventRecord(eb, h->stream);
-    T.recs.push_back({ea, eb, 1 + axis + (inv ? 0 : 3), op_bytes(h, L, S, first_pass)});
+    T.recs.push_back({ea, eb, timer_key(axis, inv, L.code, S.code), op_bytes(h, L, S, first_pass)});
   }
   return QSM_OK;
 }
@@ -362,8 +386,14 @@ int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreO
   if (rc) return rc;
   if (part && !h->dry) {
     const long nb = pass_blocks(h->fp32, h->p.nz, 2, h->geo);
-    hipError_t e = launch_finish(part, nb, h->dsc + dc_slot, h->stream);
-    if (e != hipSuccess) return fail(QSM_ERR_HIP, "dc sum: %s", hipGetErrorString(e));
+    if (h->rec) {
+      PassRec r{};
+      r.kind = 1; r.part = part; r.nparts = nb; r.out = h->dsc + dc_slot;
+      h->rec->push_back(r);
+    } else {
+      hipError_t e = launch_finish(part, nb, h->dsc + dc_slot, h->stream);
+      if (e != hipSuccess) return fail(QSM_ERR_HIP, "dc sum: %s", hipGetErrorString(e));
+    }
   }
   rc = run_pass(h, 1, false, dst_axis == 1, plain_load(tmp), plain_store(tmp), false, nullptr);
   if (rc) return rc;
@@ -864,10 +894,533 @@ extern "C" int qsm_timing_get(qsm_handle h, int which, double* total_ms, long lo
   h->timer.drain();
   double ms = 0, by = 0;
   long long c = 0;
-  for (int q = 1; q <= 6; ++q)
-    if (which == 0 || which == q) { ms += h->timer.ms[q]; by += h->timer.bytes[q]; c += h->timer.count[q]; }
+  for (const auto& kv : h->timer.acc) {
+    const int axis = kv.first / (64 * 16) / 2, inv = (kv.first / (64 * 16)) % 2;
+    const int q = 1 + axis + (inv ? 0 : 3);
+    if (which == 0 || which == q) { ms += kv.second.ms; by += kv.second.bytes; c += kv.second.count; }
+  }
   if (total_ms) *total_ms = ms;
   if (launches) *launches = c;
   if (alg_bytes) *alg_bytes = by;
+  return QSM_OK;
+}
+
+extern "C" int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, int* ld, int* st,
+                                 double* total_ms, long long* launches, double* alg_bytes) {
+  if (!h || rank < 0) return fail(QSM_ERR_ARG, "qsm_timing_kernel: bad argument");
+  h->timer.drain();
+  std::vector<std::pair<double, int>> order;
+  for (const auto& kv : h->timer.acc) order.push_back({-kv.second.ms, kv.first});
+  std::sort(order.begin(), order.end());
+  if (rank >= (int)order.size()) return fail(QSM_ERR_ARG, "qsm_timing_kernel: only %zu kernels timed", order.size());
+  const int key = order[rank].second;
+  const auto& a = h->timer.acc[key];
+  if (axis) *axis = key / (64 * 16) / 2;
+  if (inv) *inv = (key / (64 * 16)) % 2;
+  if (ld) *ld = (key / 16) % 64;
+  if (st) *st = key % 16;
+  if (total_ms) *total_ms = a.ms;
+  if (launches) *launches = a.count;
+  if (alg_bytes) *alg_bytes = a.bytes;
+  return QSM_OK;
+}
+
+// ============================================================================
+// ensembles: many independent instances (e.g. quasi-stb.cpp stability runs of
+// different grid sizes) advanced together.  Every instance records its step's
+// launches; launch q of all instances becomes one batched kernel per distinct
+// pencil length, and the whole batched step is captured as one HIP graph per
+// step signature (the same signature rules as single instances).
+
+namespace {
+
+struct EnsLaunch {
+  int kind = 0, axis = 0, n = 0, inv = 0, ld = 0, st = 0, count = 0, variant = 0;
+  long total = 0;
+  PassArgs* descs = nullptr;  // device
+  int* prefix = nullptr;      // device
+  const double** parts = nullptr;  // device (finish)
+  long* nparts = nullptr;
+  double** outs = nullptr;
+};
+
+struct EnsPlan {
+  std::vector<EnsLaunch> launches;
+  std::vector<void*> allocs;
+  hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct qsm_ens_s {
+  std::vector<qsm_handle_s*> inst;
+  int prec = 0, fp32 = 0;
+  hipStream_t stream = nullptr;
+  bool use_graphs = true;
+  std::map<std::vector<long long>, EnsPlan*> plans;
+  std::map<int, int> variant[3][2];  // tile-width variant of plain passes [axis][inv][length]
+};
+
+namespace {
+
+template <typename X>
+int upload(EnsPlan* pl, const std::vector<X>& v, X** out) {
+  void* d = nullptr;
+  HIPCHK(hipMalloc(&d, std::max<size_t>(1, v.size()) * sizeof(X)));
+  pl->allocs.push_back(d);
+  HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+  *out = (X*)d;
+  return QSM_OK;
+}
+
+int ens_issue(qsm_ens_s* e, const EnsPlan* pl) {
+  for (const EnsLaunch& L : pl->launches) {
+    hipError_t err;
+    if (L.kind == 0) {
+      BatchLaunch B{L.descs, L.prefix, L.count, L.total};
+      PassArgs dummy{};
+      err = launch_pass(e->fp32, L.n, L.axis, L.inv != 0, L.ld, L.st, dummy, nullptr, e->stream, &B,
+                        L.variant);
+    } else {
+      err = launch_finish_batch(L.parts, L.nparts, L.count, L.outs, e->stream);
+    }
+    if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble launch: %s", hipGetErrorString(err));
+  }
+  return QSM_OK;
+}
+
+int ens_build(qsm_ens_s* e, EnsPlan* pl) {
+  const int count = (int)e->inst.size();
+  std::vector<std::vector<PassRec>> recs(count);
+  for (int i = 0; i < count; ++i) {
+    e->inst[i]->rec = &recs[i];
+    int rc = one_step(e->inst[i]);
+    e->inst[i]->rec = nullptr;
+    if (rc) return rc;
+  }
+  const size_t Q = recs[0].size();
+  for (int i = 1; i < count; ++i) {
+    if (recs[i].size() != Q) return fail(QSM_ERR_STATE, "ensemble instances disagree on the step sequence");
+    for (size_t q = 0; q < Q; ++q) {
+      const PassRec &a = recs[0][q], &b = recs[i][q];
+      if (a.kind != b.kind || a.axis != b.axis || a.inv != b.inv || a.ld != b.ld || a.st != b.st)
+        return fail(QSM_ERR_STATE, "ensemble instances disagree on launch %zu", q);
+    }
+  }
+  for (size_t q = 0; q < Q; ++q) {
+    const PassRec& r0 = recs[0][q];
+    if (r0.kind == 1) {
+      std::vector<const double*> parts(count);
+      std::vector<long> np(count);
+      std::vector<double*> outs(count);
+      for (int i = 0; i < count; ++i) { parts[i] = recs[i][q].part; np[i] = recs[i][q].nparts; outs[i] = recs[i][q].out; }
+      EnsLaunch L;
+      L.kind = 1; L.count = count;
+      int rc = upload(pl, parts, &L.parts);
+      if (!rc) rc = upload(pl, np, &L.nparts);
+      if (!rc) rc = upload(pl, outs, &L.outs);
+      if (rc) return rc;
+      pl->launches.push_back(L);
+      continue;
+    }
+    std::map<int, std::vector<int>> groups;  // pencil length -> instances
+    for (int i = 0; i < count; ++i) groups[recs[i][q].n].push_back(i);
+    for (auto& g : groups) {
+      const bool plain = r0.ld == LD_PLAIN && r0.st == ST_PLAIN;
+      const int var = plain ? e->variant[r0.axis][r0.inv ? 1 : 0][g.first] : 0;
+      std::vector<PassArgs> descs;
+      std::vector<int> prefix(1, 0);
+      for (int i : g.second) {
+        const PassRec& r = recs[i][q];
+        descs.push_back(r.A);
+        prefix.push_back(prefix.back() + (int)pass_tiles(e->fp32, r.n, r.axis, r.A.geo, var));
+      }
+      EnsLaunch L;
+      L.kind = 0; L.axis = r0.axis; L.n = g.first; L.inv = r0.inv; L.ld = r0.ld; L.st = r0.st;
+      L.variant = var;
+      L.count = (int)g.second.size(); L.total = prefix.back();
+      int rc = upload(pl, descs, &L.descs);
+      if (!rc) rc = upload(pl, prefix, &L.prefix);
+      if (rc) return rc;
+      pl->launches.push_back(L);
+    }
+  }
+  return QSM_OK;
+}
+
+void ens_free_plan(EnsPlan* pl) {
+  if (pl->exec) hipGraphExecDestroy(pl->exec);
+  for (void* d : pl->allocs) hipFree(d);
+  delete pl;
+}
+
+int ens_one_step(qsm_ens_s* e) {
+  const auto key = step_key(e->inst[0]);
+  auto it = e->plans.find(key);
+  if (it != e->plans.end()) {
+    for (auto* h : e->inst) {
+      h->dry = true;
+      const int rc = one_step(h);
+      h->dry = false;
+      if (rc) return rc;
+    }
+    EnsPlan* pl = it->second;
+    if (pl->exec) { HIPCHK(hipGraphLaunch(pl->exec, e->stream)); return QSM_OK; }
+    return ens_issue(e, pl);
+  }
+  auto* pl = new EnsPlan;
+  int rc = ens_build(e, pl);  // also advances every instance's bookkeeping
+  if (rc) { ens_free_plan(pl); return rc; }
+  e->plans[key] = pl;
+  if (!e->use_graphs) return ens_issue(e, pl);
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  rc = ens_issue(e, pl);
+  hipGraph_t g = nullptr;
+  hipError_t err = hipStreamEndCapture(e->stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble capture: %s", hipGetErrorString(err));
+  err = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble graph instantiate: %s", hipGetErrorString(err));
+  HIPCHK(hipGraphLaunch(pl->exec, e->stream));
+  return QSM_OK;
+}
+
+}  // namespace
+
+extern "C" int qsm_ens_create(const qsm_params* params, int count, int prec, qsm_ens* out) {
+  if (!params || !out || count <= 0) return fail(QSM_ERR_ARG, "qsm_ens_create: bad argument");
+  *out = nullptr;
+  for (int i = 1; i < count; ++i)
+    if (params[i].variant != params[0].variant || params[i].divv_switch != params[0].divv_switch ||
+        params[i].hydro != params[0].hydro)
+      return fail(QSM_ERR_ARG, "qsm_ens_create: instances must share variant, divv_switch and hydro");
+  auto* e = new qsm_ens_s;
+  e->prec = prec;
+  e->fp32 = prec == QSM_MIXED;
+  for (int i = 0; i < count; ++i) {
+    qsm_handle h = nullptr;
+    int rc = qsm_create(&params[i], prec, &h);
+    if (rc) { qsm_ens_destroy(e); return rc; }
+    e->inst.push_back(h);
+  }
+  if (hipStreamCreate(&e->stream) != hipSuccess) { qsm_ens_destroy(e); return fail(QSM_ERR_HIP, "stream"); }
+  *out = e;
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_set_psi(qsm_ens e, int index, const double* psi) {
+  if (!e || index < 0 || index >= (int)e->inst.size()) return fail(QSM_ERR_ARG, "qsm_ens_set_psi: bad index");
+  for (auto* h : e->inst)
+    if (h->step != 0) return fail(QSM_ERR_STATE, "qsm_ens_set_psi: the ensemble has already stepped");
+  return qsm_set_psi(e->inst[index], psi);
+}
+
+extern "C" int qsm_ens_step(qsm_ens e, int nsteps) {
+  if (!e || nsteps < 0) return fail(QSM_ERR_ARG, "qsm_ens_step: bad argument");
+  for (auto* h : e->inst)
+    if (!h->ready) return fail(QSM_ERR_STATE, "qsm_ens_step: set psi of every instance first");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = ens_one_step(e);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (nsteps > 0 && e->inst[0]->p.variant == QSM_STB)
+    for (auto* h : e->inst) {
+      int rc = stb_growth(h);
+      if (rc) return rc;
+    }
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_growth(qsm_ens e, double* out) {
+  if (!e || !out) return fail(QSM_ERR_ARG, "qsm_ens_growth: null argument");
+  for (size_t i = 0; i < e->inst.size(); ++i) out[i] = e->inst[i]->growth;
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_get_field(qsm_ens e, int index, int which, double* host) {
+  if (!e || index < 0 || index >= (int)e->inst.size()) return fail(QSM_ERR_ARG, "qsm_ens_get_field: bad index");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return qsm_get_field(e->inst[index], which, host);
+}
+
+extern "C" int qsm_ens_get_diag(qsm_ens e, int index, qsm_diag* d) {
+  if (!e || index < 0 || index >= (int)e->inst.size()) return fail(QSM_ERR_ARG, "qsm_ens_get_diag: bad index");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return qsm_get_diag(e->inst[index], d);
+}
+
+extern "C" int qsm_ens_set_graphs(qsm_ens e, int enable) {
+  if (!e) return fail(QSM_ERR_ARG, "qsm_ens_set_graphs: null ensemble");
+  e->use_graphs = enable != 0;
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_destroy(qsm_ens e) {
+  if (!e) return QSM_OK;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (auto& kv : e->plans) ens_free_plan(kv.second);
+  for (auto* h : e->inst) qsm_destroy(h);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return QSM_OK;
+}
+
+// ============================================================================
+// full-state checkpoint.  The reference restarts from psi<rank>.dat text files
+// (6 significant digits, psi only: quasi.cpp:692-717, 2051-2062), which loses
+// psihat_old, N_old, the lagged rhohat, rho_old, v and the step count.  This
+// writes every buffer the step reads, in logical order, so a resumed run
+// continues bit for bit.
+
+namespace {
+
+const char kCkptMagic[8] = {'Q', 'S', 'M', 'C', 'K', 'P', 'T', '1'};
+
+struct CkptHeader {
+  char magic[8];
+  qsm_params p;
+  int prec;
+  int pad_;
+  long long step;
+  long long npts;
+};
+
+}  // namespace
+
+extern "C" int qsm_checkpoint_write(qsm_handle h, const char* path) {
+  if (!h || !path) return fail(QSM_ERR_ARG, "qsm_checkpoint_write: null argument");
+  if (!h->ready) return fail(QSM_ERR_STATE, "qsm_checkpoint_write: no state");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(QSM_ERR_ARG, "qsm_checkpoint_write: cannot open %s", path);
+  CkptHeader hd{};
+  memcpy(hd.magic, kCkptMagic, 8);
+  hd.p = h->p; hd.prec = h->prec; hd.step = h->step; hd.npts = (long long)h->n;
+  bool ok = fwrite(&hd, sizeof hd, 1, fp) == 1;
+  std::vector<char> buf(h->n * 8);
+  auto put = [&](const void* d, size_t bytes) {
+    if (!ok) return;
+    ok = hipMemcpy(buf.data(), d, bytes, hipMemcpyDeviceToHost) == hipSuccess &&
+         fwrite(buf.data(), 1, bytes, fp) == bytes;
+  };
+  put(h->psihat, h->n * 8);
+  put(h->psihat_old, h->n * 8);
+  put(h->dsc, D_COUNT * 8);
+  const int order[] = {h->psi_cur, h->psi_old, h->rho_cur, h->rho_old};
+  for (int f : order) put(h->f[f], h->n * h->esz);
+  for (int f = 0; f < F_COUNT; ++f)
+    if (f != F_PSI_A && f != F_PSI_B && f != F_RHO_A && f != F_RHO_B) put(h->f[f], h->n * h->esz);
+  ok = (fclose(fp) == 0) && ok;
+  return ok ? QSM_OK : fail(QSM_ERR_HIP, "qsm_checkpoint_write: write failed (%s)", path);
+}
+
+extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
+  if (!h || !path) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: null argument");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: cannot open %s", path);
+  CkptHeader hd{};
+  bool ok = fread(&hd, sizeof hd, 1, fp) == 1 && memcmp(hd.magic, kCkptMagic, 8) == 0;
+  if (!ok) { fclose(fp); return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is not a checkpoint", path); }
+  if (memcmp(&hd.p, &h->p, sizeof(qsm_params)) != 0 || hd.prec != h->prec || hd.npts != (long long)h->n) {
+    fclose(fp);
+    return fail(QSM_ERR_ARG, "qsm_checkpoint_read: parameters or precision differ from the handle's");
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<char> buf(h->n * 8);
+  auto get = [&](void* d, size_t bytes) {
+    if (!ok) return;
+    ok = fread(buf.data(), 1, bytes, fp) == bytes &&
+         hipMemcpy(d, buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  get(h->psihat, h->n * 8);
+  get(h->psihat_old, h->n * 8);
+  get(h->dsc, D_COUNT * 8);
+  const int order[] = {h->psi_cur, h->psi_old, h->rho_cur, h->rho_old};
+  for (int f : order) get(h->f[f], h->n * h->esz);
+  for (int f = 0; f < F_COUNT; ++f)
+    if (f != F_PSI_A && f != F_PSI_B && f != F_RHO_A && f != F_RHO_B) get(h->f[f], h->n * h->esz);
+  fclose(fp);
+  if (!ok) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is truncated", path);
+  h->step = hd.step;
+  h->ready = true;
+  if (h->p.variant == QSM_STB) return stb_growth(h);
+  return QSM_OK;
+}
+
+// ============================================================================
+// autotuner: for every plain pass kind (axis x direction) of a handle, or
+// every plain batched pass group (axis x direction x length) of an ensemble,
+// time both tile-width variants on the scratch field (never the state) with
+// HIP events and keep the faster.  Choices are cached in a flat JSON object
+// {"<gpu>|<precision>|<grid>|<axis>|<dir>": variant} when a path is given.
+
+namespace {
+
+std::map<std::string, int> cache_read(const char* path) {
+  std::map<std::string, int> m;
+  if (!path) return m;
+  FILE* fp = fopen(path, "r");
+  if (!fp) return m;
+  std::string txt;
+  char buf[4096];
+  size_t r;
+  while ((r = fread(buf, 1, sizeof buf, fp)) > 0) txt.append(buf, r);
+  fclose(fp);
+  size_t pos = 0;
+  while ((pos = txt.find('"', pos)) != std::string::npos) {
+    const size_t end = txt.find('"', pos + 1);
+    if (end == std::string::npos) break;
+    const std::string key = txt.substr(pos + 1, end - pos - 1);
+    const size_t colon = txt.find(':', end);
+    if (colon == std::string::npos) break;
+    m[key] = atoi(txt.c_str() + colon + 1);
+    pos = txt.find_first_of(",}", colon);
+    if (pos == std::string::npos) break;
+  }
+  return m;
+}
+
+void cache_write(const char* path, const std::map<std::string, int>& m) {
+  if (!path) return;
+  FILE* fp = fopen(path, "w");
+  if (!fp) return;
+  fprintf(fp, "{\n");
+  size_t i = 0;
+  for (const auto& kv : m) fprintf(fp, "  \"%s\": %d%s\n", kv.first.c_str(), kv.second, ++i < m.size() ? "," : "");
+  fprintf(fp, "}\n");
+  fclose(fp);
+}
+
+std::string gpu_name() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "gpu";
+  std::string s = prop.gcnArchName;
+  for (char& c : s) if (c == '|' || c == '"' || c == ':') c = '_';
+  return s;
+}
+
+// average ms of `reps` launches of fn on stream s (after one warm-up launch)
+template <class F>
+double time_launches(hipStream_t s, int reps, F fn) {
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  fn();
+  (void)hipEventRecord(a, s);
+  for (int r = 0; r < reps; ++r) fn();
+  (void)hipEventRecord(b, s);
+  (void)hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+}  // namespace
+
+extern "C" int qsm_autotune(qsm_handle h, const char* cache_path) {
+  if (!h) return fail(QSM_ERR_ARG, "qsm_autotune: null handle");
+  auto cache = cache_read(cache_path);
+  const std::string base = gpu_name() + "|" + (h->fp32 ? "f32" : "f64") + "|" + std::to_string(h->p.nx) +
+                           "x" + std::to_string(h->p.ny) + "x" + std::to_string(h->p.nz);
+  const int nn[3] = {h->p.nx, h->p.ny, h->p.nz};
+  void* scratch = h->f[F_NTMP];  // a transform intermediate: rewritten before every use
+  for (int axis = 0; axis < 3; ++axis)
+    for (int inv = 0; inv < 2; ++inv) {
+      const std::string key = base + "|" + std::to_string(axis) + "|" + (inv ? "inv" : "fwd");
+      auto it = cache.find(key);
+      if (it != cache.end()) { h->variant[axis][inv] = it->second; continue; }
+      PassArgs A{};
+      A.geo = h->geo; A.waves = h->waves; A.phys = h->phys; A.tw = h->tw[axis];
+      A.ld.code = LD_PLAIN; A.ld.a[0] = scratch; A.st.code = ST_PLAIN; A.st.d[0] = scratch;
+      double best = 1e30;
+      int bv = 0;
+      for (int v = 0; v < 2; ++v) {
+        hipError_t err = hipSuccess;
+        const double ms = time_launches(h->stream, 5, [&] {
+          hipError_t e2 = launch_pass(h->fp32, nn[axis], axis, inv != 0, LD_PLAIN, ST_PLAIN, A, nullptr,
+                                      h->stream, nullptr, v);
+          if (e2 != hipSuccess) err = e2;
+        });
+        if (err != hipSuccess) return fail(QSM_ERR_HIP, "qsm_autotune: %s", hipGetErrorString(err));
+        if (ms < best) { best = ms; bv = v; }
+      }
+      h->variant[axis][inv] = bv;
+      cache[key] = bv;
+    }
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+  h->graphs.clear();
+  cache_write(cache_path, cache);
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
+  if (!e) return fail(QSM_ERR_ARG, "qsm_ens_autotune: null ensemble");
+  auto cache = cache_read(cache_path);
+  // lengths present on each axis, with the instances that have them
+  std::map<int, std::vector<int>> by_n[3];
+  for (int i = 0; i < (int)e->inst.size(); ++i) {
+    const qsm_params& p = e->inst[i]->p;
+    by_n[0][p.nx].push_back(i); by_n[1][p.ny].push_back(i); by_n[2][p.nz].push_back(i);
+  }
+  const std::string base = gpu_name() + "|" + (e->fp32 ? "f32" : "f64") + "|ens" + std::to_string(e->inst.size());
+  for (int axis = 0; axis < 3; ++axis)
+    for (auto& g : by_n[axis])
+      for (int inv = 0; inv < 2; ++inv) {
+        const std::string key = base + "|" + std::to_string(g.first) + "|" + std::to_string(axis) + "|" + (inv ? "inv" : "fwd");
+        auto it = cache.find(key);
+        if (it != cache.end()) { e->variant[axis][inv][g.first] = it->second; continue; }
+        double best = 1e30;
+        int bv = 0;
+        for (int v = 0; v < 2; ++v) {
+          std::vector<PassArgs> descs;
+          std::vector<int> prefix(1, 0);
+          for (int i : g.second) {
+            qsm_handle_s* h = e->inst[i];
+            PassArgs A{};
+            A.geo = h->geo; A.waves = h->waves; A.phys = h->phys; A.tw = h->tw[axis];
+            A.ld.code = LD_PLAIN; A.ld.a[0] = h->f[F_NTMP]; A.st.code = ST_PLAIN; A.st.d[0] = h->f[F_NTMP];
+            descs.push_back(A);
+            prefix.push_back(prefix.back() + (int)pass_tiles(e->fp32, g.first, axis, h->geo, v));
+          }
+          EnsPlan tmp;
+          PassArgs* dd = nullptr;
+          int* dp = nullptr;
+          int rc = upload(&tmp, descs, &dd);
+          if (!rc) rc = upload(&tmp, prefix, &dp);
+          if (rc) { for (void* d : tmp.allocs) (void)hipFree(d); return rc; }
+          BatchLaunch B{dd, dp, (int)descs.size(), prefix.back()};
+          hipError_t err = hipSuccess;
+          const double ms = time_launches(e->stream, 5, [&] {
+            PassArgs dummy{};
+            hipError_t e2 = launch_pass(e->fp32, g.first, axis, inv != 0, LD_PLAIN, ST_PLAIN, dummy, nullptr,
+                                        e->stream, &B, v);
+            if (e2 != hipSuccess) err = e2;
+          });
+          (void)hipStreamSynchronize(e->stream);
+          for (void* d : tmp.allocs) (void)hipFree(d);
+          tmp.allocs.clear();
+          if (err != hipSuccess) return fail(QSM_ERR_HIP, "qsm_ens_autotune: %s", hipGetErrorString(err));
+          if (ms < best) { best = ms; bv = v; }
+        }
+        e->variant[axis][inv][g.first] = bv;
+        cache[key] = bv;
+      }
+  for (auto& kv : e->plans) ens_free_plan(kv.second);
+  e->plans.clear();
+  cache_write(cache_path, cache);
+  return QSM_OK;
+}
+
+extern "C" int qsm_set_variant(qsm_handle h, int axis, int inv, int variant) {
+  if (!h || axis < 0 || axis > 2 || (inv != 0 && inv != 1) || (variant != 0 && variant != 1))
+    return fail(QSM_ERR_ARG, "qsm_set_variant: bad argument");
+  h->variant[axis][inv] = variant;
+  for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+  h->graphs.clear();
   return QSM_OK;
 }

@@ -262,16 +262,19 @@ struct StIn {
 
 template <int ST>
 constexpr bool store_reads() {
-  return ST == ST_UPDATE || ST == ST_UPDATE_STB;
+  return ST == ST_UPDATE || ST == ST_UPDATE_STB || ST == ST_FILTER;
 }
 
 template <typename T, int ST>
-__device__ __forceinline__ StIn store_inputs(const StoreOp& S, long idx) {
+__device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Waves& W, int i, int j, int k,
+                                             long idx) {
   StIn in{0.0, 0.0, 0.0};
-  if constexpr (store_reads<ST>()) {
+  if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
     in.ph = ((const double*)S.s[0])[idx];
     in.pho = ((const double*)S.d[0])[idx];
     in.nold = (double)((const T*)S.d[1])[idx];
+  } else if constexpr (ST == ST_FILTER) {
+    in.ph = W.fx[i] * W.fy[j] * W.fz[k];
   }
   return in;
 }
@@ -281,6 +284,15 @@ __device__ __forceinline__ StIn store_inputs(const StoreOp& S, long idx) {
 template <typename T, int ST>
 __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Waves& W, int i,
                                             int j, int k, long idx, T v, const StIn& in) {
+  if constexpr (ST == ST_FILTER) {
+    // quasi.cpp:1260-1261 with exp(-filter K^2) = fx[i] fy[j] fz[k] (in.ph)
+    double raw = (double)v;
+    if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
+    const double r = P.scale * raw * in.ph;
+    ((T*)S.d[0])[idx] = (T)r;
+    if (S.dc_out && idx == 0) S.dc_out[0] = r;
+    return;
+  }
   const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
   const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
   double nh = (double)v;
@@ -305,9 +317,10 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
 // evaluated in chunks of 8 elements (register pressure); the rest in one go.
 template <int LD>
 constexpr bool load_heavy() {
-  return LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_PRESSURE ||
-         LD == LD_DIVV || LD == LD_NONLIN || LD == LD_NONLIN_P || LD == LD_STRESS ||
-         LD == LD_RHOSTAR || LD == LD_STRESS_STB || LD == LD_NONLIN_STB;
+  // the x-pass projection operators: FP64 arithmetic on four fields in a
+  // 512-thread block (128-VGPR budget); the real-space operators of the
+  // z pass run in 256-thread blocks and stay fully unrolled
+  return LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_PRESSURE;
 }
 
 template <typename T, int ST>

@@ -7,8 +7,9 @@
 namespace qsm {
 
 #define QSM_DECL(T, AX) \
-  hipError_t launch_pass_##T##_##AX(int n, bool inv, int ld, int st, const PassArgs& A, long* nb, \
-                                    hipStream_t s, const BatchLaunch* B);
+  hipError_t launch_pass_##T##_##AX(int n, bool inv, int ld, int st, int variant,          \
+                                    const PassArgs& A, long* nb, hipStream_t s,                 \
+                                    const BatchLaunch* B);
 QSM_DECL(float, 0)
 QSM_DECL(float, 1)
 QSM_DECL(float, 2)
@@ -18,18 +19,24 @@ QSM_DECL(double, 2)
 #undef QSM_DECL
 
 hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
-                       long* nb, hipStream_t s, const BatchLaunch* B) {
+                       long* nb, hipStream_t s, const BatchLaunch* B, int variant) {
+  const int v = (ld == LD_PLAIN && st == ST_PLAIN) ? variant : 0;
   if (prec_fp32) {
-    if (axis == 0) return launch_pass_float_0(n, inv, ld, st, A, nb, s, B);
-    if (axis == 1) return launch_pass_float_1(n, inv, ld, st, A, nb, s, B);
-    return launch_pass_float_2(n, inv, ld, st, A, nb, s, B);
+    if (axis == 0) return launch_pass_float_0(n, inv, ld, st, v, A, nb, s, B);
+    if (axis == 1) return launch_pass_float_1(n, inv, ld, st, v, A, nb, s, B);
+    return launch_pass_float_2(n, inv, ld, st, v, A, nb, s, B);
   }
-  if (axis == 0) return launch_pass_double_0(n, inv, ld, st, A, nb, s, B);
-  if (axis == 1) return launch_pass_double_1(n, inv, ld, st, A, nb, s, B);
-  return launch_pass_double_2(n, inv, ld, st, A, nb, s, B);
+  if (axis == 0) return launch_pass_double_0(n, inv, ld, st, v, A, nb, s, B);
+  if (axis == 1) return launch_pass_double_1(n, inv, ld, st, v, A, nb, s, B);
+  return launch_pass_double_2(n, inv, ld, st, v, A, nb, s, B);
 }
 
-long pass_tiles(int prec_fp32, int n, int axis, const Geo& g) { return pass_blocks(prec_fp32, n, axis, g); }
+long pass_tiles(int prec_fp32, int n, int axis, const Geo& g, int variant) {
+  const int tk = pass_tile_k(prec_fp32, n, axis, variant);
+  if (tk <= 0) return -1;
+  if (axis == kAxZ) return ((long)g.nx * g.ny + tk - 1) / tk;
+  return (axis == kAxY ? (long)g.nx : (long)g.ny) * ((g.nz + tk - 1) / tk);
+}
 
 bool step_supported(int n) { return QSM_IS_STEP_N(n); }
 
@@ -43,13 +50,13 @@ bool pass_supported(int n) {
   }
 }
 
-template <typename T>
+template <typename T, int V>
 static int tile_k_t(int n, int axis) {
   switch (n) {
-#define QSM_CASE(NN)                                            \
-  case NN:                                                      \
-    return axis == kAxZ ? TileCfg<T, NN, kAxZ>::TK              \
-                        : (axis == kAxY ? TileCfg<T, NN, kAxY>::TK : TileCfg<T, NN, kAxX>::TK);
+#define QSM_CASE(NN)                                                  \
+  case NN:                                                            \
+    return axis == kAxZ ? TileCfg<T, NN, kAxZ, V>::TK                 \
+                        : (axis == kAxY ? TileCfg<T, NN, kAxY, V>::TK : TileCfg<T, NN, kAxX, V>::TK);
     QSM_FOR_EACH_N(QSM_CASE)
 #undef QSM_CASE
     default:
@@ -57,8 +64,9 @@ static int tile_k_t(int n, int axis) {
   }
 }
 
-int pass_tile_k(int prec_fp32, int n, int axis) {
-  return prec_fp32 ? tile_k_t<float>(n, axis) : tile_k_t<double>(n, axis);
+int pass_tile_k(int prec_fp32, int n, int axis, int variant) {
+  if (variant == 1) return prec_fp32 ? tile_k_t<float, 1>(n, axis) : tile_k_t<double, 1>(n, axis);
+  return prec_fp32 ? tile_k_t<float, 0>(n, axis) : tile_k_t<double, 0>(n, axis);
 }
 
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g) {

@@ -31,7 +31,11 @@ constexpr int kAxX = 0, kAxY = 1, kAxZ = 2;
 
 
 
-template <typename T, int N, int AX>
+// Tile configuration of a pass.  V selects the tile-width variant: V = 0 the
+// default width below, V = 1 half of it (64-byte row segments on strided axes,
+// half the LDS per workgroup, twice the resident workgroups per CU).  The
+// autotuner picks a variant per (axis, direction) for the plain passes.
+template <typename T, int N, int AX, int V = 0>
 struct TileCfg {
   static constexpr int M = N / 2;
   static constexpr int E = Plan<M>::E;
@@ -49,9 +53,10 @@ struct TileCfg {
     while (t > 1 && (long)M * (t + 1) * 2 * (long)sizeof(T) > 72000) t /= 2;
     return t;
   }
-  static constexpr int TK = tk();
+  static constexpr int TK = (V == 1 && tk() > 1) ? tk() / 2 : tk();
   static constexpr int TKP = TK + 1;  // LDS slot stride: odd, so row-wise z-pass access spreads banks
   static constexpr int BLOCK = TK * TP;
+  static_assert(BLOCK % TK == 0, "a thread's column is fixed across its loads");
 };
 
 // position of sample n in the Makhoul-reordered array v, and its inverse
@@ -93,9 +98,9 @@ struct TileGeo {
 };
 
 // One workgroup's work: tile `tile` of the pass described by A.
-template <typename T, int N, int AX, bool INV, int LD, int ST>
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
-  using C = TileCfg<T, N, AX>;
+  using C = TileCfg<T, N, AX, V>;
   using PL = Plan<N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1, NB1 = E / R1;
@@ -129,9 +134,11 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   cx<T> v[E];
 
   if constexpr (!INV) {
-    if constexpr (AX == kAxZ) {
-      // coalesced row loads into LDS in Makhoul order; the loads of a chunk
-      // are all issued before any of its values is used
+    {
+      // row-sweeping loads into LDS in Makhoul order (a wavefront reads 64
+      // consecutive elements of a z row, or TK-column segments of
+      // consecutive rows of a strided axis); the loads of a chunk are all
+      // issued before any of its values is used
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
 #pragma unroll 1
@@ -140,15 +147,18 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
-          tg.at(e / N, e % N, i, j, k, idx);
+          tg.at(q, n, i, j, k, idx);
           xs[c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
-          const int q = e / N, n = e % N;
+          int q, n;
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
           T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
@@ -160,42 +170,6 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
       }
       __syncthreads();
       gather<T, M, E, TKP, R1>(v, buf, p, t);
-    } else {
-      // strided: each thread loads its stage-1 inputs straight from HBM; a
-      // wavefront reads TK consecutive columns of one row per load
-      int i, j, k;
-      long idx;
-      const bool ok = tg.at(p, 0, i, j, k, idx);
-      T xs[2 * E];
-#pragma unroll
-      for (int b = 0; b < NB1; ++b) {
-#pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          const int m = t + b * TP + r * (M / R1);
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            tg.at(p, xpos<N>(2 * m + c), i, j, k, idx);
-            xs[2 * (b * R1 + r) + c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
-          }
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < NB1; ++b) {
-#pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          const int m = t + b * TP + r * (M / R1);
-          T xr[2];
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            const int n = xpos<N>(2 * m + c);
-            T x = ok ? xs[2 * (b * R1 + r) + c] : (T)0;
-            if (A.part) dsum += (double)x;
-            xr[c] = (dst && (n & 1)) ? -x : x;
-          }
-          v[b * R1 + r] = cx<T>{xr[0], xr[1]};
-        }
-      }
-      __syncthreads();  // twiddle table
     }
     run_fft<T, N, M, E, TKP, -1, true>(typename PL::R{}, v, buf, p, t, tq);
 
@@ -253,17 +227,17 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
           int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
           tg.at(q, n, i, j, k, idx);
-          in[c] = store_inputs<T, ST>(A.st, idx);
+          in[c] = store_inputs<T, ST>(A.st, A.waves, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
           int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
           if (!tg.at(q, n, i, j, k, idx)) continue;
@@ -285,7 +259,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
           int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
           tg.at(q, n, i, j, k, idx);
@@ -295,7 +269,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
         for (int c = 0; c < CH; ++c) {
           const int e = tid + (s0 + c) * BLOCK;
           int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = e % TK; n = e / TK; }
+          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
           int i, j, k;
           long idx;
           const T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
@@ -383,16 +357,16 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   }
 }
 
-template <typename T, int N, int AX, bool INV, int LD, int ST>
-__global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK)) k_pass(const PassArgs A) {
-  pass_tile<T, N, AX, INV, LD, ST>(A, (int)blockIdx.x);
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
+__global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const PassArgs A) {
+  pass_tile<T, N, AX, INV, LD, ST, V>(A, (int)blockIdx.x);
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
 // length N: workgroup b runs tile b - prefix[q] of instance q, where
 // prefix[q] <= b < prefix[q+1] (prefix[0] = 0, prefix[count] = grid size).
-template <typename T, int N, int AX, bool INV, int LD, int ST>
-__global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
+__global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
     k_pass_batch(const PassArgs* __restrict__ descs, const int* __restrict__ prefix, int count) {
   const int b = (int)blockIdx.x;
   int lo = 0, hi = count;  // invariant: prefix[lo] <= b < prefix[hi]
@@ -400,32 +374,32 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX>::BLOCK))
     const int mid = (lo + hi) >> 1;
     if (prefix[mid] <= b) lo = mid; else hi = mid;
   }
-  pass_tile<T, N, AX, INV, LD, ST>(descs[lo], b - prefix[lo]);
+  pass_tile<T, N, AX, INV, LD, ST, V>(descs[lo], b - prefix[lo]);
 }
 
 
-template <typename T, int N, int AX>
+template <typename T, int N, int AX, int V = 0>
 long tiles_of(const Geo& g) {
-  using C = TileCfg<T, N, AX>;
+  using C = TileCfg<T, N, AX, V>;
   if (AX == kAxZ) return ((long)g.nx * g.ny + C::TK - 1) / C::TK;
   const long tiles_k = (g.nz + C::TK - 1) / C::TK;
   return (AX == kAxY ? (long)g.nx : (long)g.ny) * tiles_k;
 }
 
 // Single pass (B == nullptr) or batched pass (B: device descriptors).
-template <typename T, int N, int AX, bool INV, int LD, int ST>
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V = 0>
 hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const BatchLaunch* B) {
-  using C = TileCfg<T, N, AX>;
+  using C = TileCfg<T, N, AX, V>;
   if (B) {
     if (B->total_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_pass_batch<T, N, AX, INV, LD, ST>), dim3((unsigned)B->total_tiles),
+    hipLaunchKernelGGL((k_pass_batch<T, N, AX, INV, LD, ST, V>), dim3((unsigned)B->total_tiles),
                        dim3(C::BLOCK), 0, s, B->descs, B->prefix, B->count);
     return hipGetLastError();
   }
-  const long nb = tiles_of<T, N, AX>(A.geo);
+  const long nb = tiles_of<T, N, AX, V>(A.geo);
   if (nblocks_out) *nblocks_out = nb;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_pass<T, N, AX, INV, LD, ST>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
+  hipLaunchKernelGGL((k_pass<T, N, AX, INV, LD, ST, V>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
   return hipGetLastError();
 }
 
@@ -435,11 +409,15 @@ hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const
 // forward z passes load the real-space operators, forward x passes store the
 // spectral epilogues, inverse x passes load the spectral prologues.
 template <typename T, int AX, int N, bool STEP>
-hipError_t dispatch_ops(bool inv, int ld, int st, const PassArgs& A, long* nb, hipStream_t s,
-                        const BatchLaunch* B) {
-  if (ld == LD_PLAIN && st == ST_PLAIN)
+hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A, long* nb,
+                        hipStream_t s, const BatchLaunch* B) {
+  if (ld == LD_PLAIN && st == ST_PLAIN) {
+    if (variant == 1)
+      return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN, 1>(A, nb, s, B)
+                 : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN, 1>(A, nb, s, B);
     return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN>(A, nb, s, B)
                : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN>(A, nb, s, B);
+  }
   if constexpr (STEP && AX == kAxZ) {
     if (!inv && st == ST_PLAIN) {
       switch (ld) {

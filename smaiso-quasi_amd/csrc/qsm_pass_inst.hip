@@ -15,12 +15,13 @@
 
 namespace qsm {
 
-hipError_t QSM_CAT(launch_pass, QSM_T, QSM_AX)(int n, bool inv, int ld, int st, const PassArgs& A,
-                                               long* nb, hipStream_t s, const BatchLaunch* B) {
+hipError_t QSM_CAT(launch_pass, QSM_T, QSM_AX)(int n, bool inv, int ld, int st, int variant,
+                                               const PassArgs& A, long* nb, hipStream_t s,
+                                               const BatchLaunch* B) {
   switch (n) {
 #define QSM_CASE(NN) \
   case NN:           \
-    return dispatch_ops<QSM_T, QSM_AX, NN, QSM_IS_STEP_N(NN)>(inv, ld, st, A, nb, s, B);
+    return dispatch_ops<QSM_T, QSM_AX, NN, QSM_IS_STEP_N(NN)>(inv, ld, st, variant, A, nb, s, B);
     QSM_FOR_EACH_N(QSM_CASE)
 #undef QSM_CASE
     default:

@@ -136,3 +136,32 @@ def stb_psi(shape, amp=math.sqrt(0.12), q0=1.0, nw=8, qi=0.5, phi=0.1):
     psi = (amp * np.cos(q0 * k * dx) + (amp * phi / 2) * np.cos(q0 * k * dx + qi * i * dx)
            - (amp * phi / 2) * np.cos(q0 * k * dx - qi * i * dx))
     return np.ascontiguousarray(np.broadcast_to(psi, shape))
+
+
+# stability ensemble (quasi-stb.cpp workload): pure smectic, CN/AB2 step,
+# constants of quasi-stb.cpp:149-220 with eps near coexistence
+# (eps_c = -0.16875 beta^2/gamma = -0.009 for beta = 0.4, gamma = 3, where the
+# reference's Amp = sqrt(0.9 beta/gamma) = sqrt(0.12) is the one-mode amplitude)
+STB_BASE = dict(
+    variant=1, compat_ranks=0, nw=8, divv_switch=200, hydro=1,
+    q0=1.0, alpha=1.0, beta=0.4, gamma=3.0, eps=-0.009, nu=0.1,
+    kp=0.5, rho0=0.1, amp=math.sqrt(0.12), rho_m=0.5, dt=0.0005, filter=1.57 * 1.57 * 8,
+    growth_q=0.5,
+)
+ENSEMBLE_EDGES = (32, 48, 64, 96, 128)
+
+
+def ensemble_members(count=96, seed=4, pert_amp=1e-3):
+    """`count` instances: each grid edge drawn uniformly from ENSEMBLE_EDGES
+    per axis, perturbation wavenumber Qi ~ U(0.05, 1.0), perturbation amplitude
+    A phi / 2 = pert_amp.  Returns [(params dict, psi)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx, ny, nz = (int(v) for v in rng.choice(ENSEMBLE_EDGES, size=3))
+        qi = float(rng.uniform(0.05, 1.0))
+        d = dict(STB_BASE)
+        d.update(nx=nx, ny=ny, nz=nz, growth_q=qi)
+        phi = 2 * pert_amp / d["amp"]
+        out.append((d, stb_psi((nx, ny, nz), amp=d["amp"], qi=qi, phi=phi)))
+    return out

@@ -81,8 +81,23 @@ SIGNATURES = {
     "qsm_get_field": ([_VP, _IP, _DP], _IP),
     "qsm_get_diag": ([_VP, ctypes.POINTER(Diag)], _IP),
     "qsm_destroy": ([_VP], _IP),
+    "qsm_checkpoint_write": ([_VP, ctypes.c_char_p], _IP),
+    "qsm_checkpoint_read": ([_VP, ctypes.c_char_p], _IP),
+    "qsm_ens_create": ([ctypes.POINTER(Params), _IP, _IP, ctypes.POINTER(_VP)], _IP),
+    "qsm_ens_set_psi": ([_VP, _IP, _DP], _IP),
+    "qsm_ens_step": ([_VP, _IP], _IP),
+    "qsm_ens_growth": ([_VP, _DP], _IP),
+    "qsm_ens_get_field": ([_VP, _IP, _IP, _DP], _IP),
+    "qsm_ens_get_diag": ([_VP, _IP, ctypes.POINTER(Diag)], _IP),
+    "qsm_ens_set_graphs": ([_VP, _IP], _IP),
+    "qsm_ens_destroy": ([_VP], _IP),
+    "qsm_autotune": ([_VP, ctypes.c_char_p], _IP),
+    "qsm_ens_autotune": ([_VP, ctypes.c_char_p], _IP),
+    "qsm_set_variant": ([_VP, _IP, _IP, _IP], _IP),
     "qsm_timing_enable": ([_VP, _IP], _IP),
     "qsm_timing_get": ([_VP, _IP, _DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
+    "qsm_timing_kernel": ([_VP, _IP] + [ctypes.POINTER(_IP)] * 4 +
+                          [_DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
 }
 
 _lib = None
@@ -203,6 +218,18 @@ class Sim:
         _check(lib().qsm_get_diag(self.h, ctypes.byref(d)))
         return {n: getattr(d, n) for n, _ in Diag._fields_}
 
+    def autotune(self, cache_path=None):
+        _check(lib().qsm_autotune(self.h, os.fsencode(cache_path) if cache_path else None))
+
+    def set_variant(self, axis, inv, variant):
+        _check(lib().qsm_set_variant(self.h, axis, inv, variant))
+
+    def checkpoint_write(self, path):
+        _check(lib().qsm_checkpoint_write(self.h, os.fsencode(path)))
+
+    def checkpoint_read(self, path):
+        _check(lib().qsm_checkpoint_read(self.h, os.fsencode(path)))
+
     def timing(self, enable=True):
         _check(lib().qsm_timing_enable(self.h, 1 if enable else 0))
 
@@ -213,3 +240,69 @@ class Sim:
         _check(lib().qsm_timing_get(self.h, which, ctypes.byref(ms), ctypes.byref(cnt),
                                     ctypes.byref(by)))
         return {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
+
+    def timing_kernel(self, rank=0):
+        """rank-th kernel by device time: {axis, inv, ld, st, ms, launches, bytes}"""
+        ax, inv, ld, st = (ctypes.c_int() for _ in range(4))
+        ms, cnt, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        _check(lib().qsm_timing_kernel(self.h, rank, ctypes.byref(ax), ctypes.byref(inv),
+                                       ctypes.byref(ld), ctypes.byref(st), ctypes.byref(ms),
+                                       ctypes.byref(cnt), ctypes.byref(by)))
+        return {"axis": ax.value, "inv": inv.value, "ld": ld.value, "st": st.value,
+                "ms": ms.value, "launches": cnt.value, "bytes": by.value}
+
+
+class Ensemble:
+    """Independent instances advanced together (qsm_ens_*): one batched
+    kernel per pencil length per launch of the step, one graph per step."""
+
+    def __init__(self, params_list, prec=FP64):
+        self.params = list(params_list)
+        self.prec = prec
+        arr = (Params * len(self.params))(*self.params)
+        h = ctypes.c_void_p()
+        _check(lib().qsm_ens_create(arr, len(self.params), prec, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().qsm_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, i):
+        p = self.params[i]
+        return (p.nx, p.ny, p.nz)
+
+    def set_psi(self, i, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(self.shape(i))
+        _check(lib().qsm_ens_set_psi(self.h, i, _dp(psi)))
+
+    def step(self, n=1):
+        _check(lib().qsm_ens_step(self.h, n))
+
+    def graphs(self, enable=True):
+        _check(lib().qsm_ens_set_graphs(self.h, 1 if enable else 0))
+
+    def autotune(self, cache_path=None):
+        _check(lib().qsm_ens_autotune(self.h, os.fsencode(cache_path) if cache_path else None))
+
+    def growth(self):
+        out = np.empty(len(self.params))
+        _check(lib().qsm_ens_growth(self.h, _dp(out)))
+        return out
+
+    def field(self, i, name):
+        out = np.empty(self.shape(i))
+        _check(lib().qsm_ens_get_field(self.h, i, FIELD[name], _dp(out)))
+        return out
+
+    def diag(self, i):
+        d = Diag()
+        _check(lib().qsm_ens_get_diag(self.h, i, ctypes.byref(d)))
+        return {n: getattr(d, n) for n, _ in Diag._fields_}

@@ -208,3 +208,65 @@ def test_graph_replay_matches_eager(qsm_mod, prec):
         for f in ("PSI", "RHO", "VX", "PSIHAT", "NHAT"):
             assert np.array_equal(g.field(f), e.field(f)), (chunk, f)
         assert g.diag() == e.diag()
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec):
+    """A batched ensemble of stb instances with mixed grid edges gives, for
+    every instance, the fields and growth rate of that instance run alone
+    through the single-instance path to round-off (the batched entry point is
+    a different kernel, so the compiler may contract a different set of
+    multiply-adds: ~1 ulp), and FP64 members match the oracle."""
+    import ics
+    members = ics.ensemble_members(count=6, seed=11)
+    params = [qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_}) for d, _ in members]
+    ens = qsm_mod.Ensemble(params, prec)
+    for i, (_, psi) in enumerate(members):
+        ens.set_psi(i, psi)
+    ens.step(2)
+    ens.step(3)
+    g = ens.growth()
+    for i, (d, psi) in enumerate(members):
+        print(f"member {i} shape {ens.shape(i)} Qi={d['growth_q']:.3f} growth={g[i]:.6e}", flush=True)
+        sim = qsm_mod.Sim(params[i], prec)
+        sim.set_psi(psi)
+        sim.step(5)
+        tol = 1e-13 if prec == 0 else 1e-5
+        for f in ("PSI", "PSIHAT", "VX", "VZ", "NHAT"):
+            assert rel_l2(ens.field(i, f), sim.field(f)) < tol, (i, f)
+        # the growth rate (psihat_new - psihat_old)/(dt psihat_old) amplifies ulps by ~1/(dt g)
+        assert abs(g[i] - sim.diag()["growth"]) <= (1e-9 if prec == 0 else 1e-3) * max(1.0, abs(g[i])), i
+        if prec == 0 and i < 2:
+            orc = oracle_mod.State(oracle_mod.make_params(**d))
+            orc.set_psi(psi)
+            orc.step(5)
+            assert rel_l2(ens.field(i, "PSI"), orc.field("PSI")) < 1e-12
+            assert abs(g[i] - orc.growth) <= 1e-9 * max(1.0, abs(orc.growth))
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_tile_variants_are_bit_identical(qsm_mod, prec):
+    """The half-width tile variant of every plain pass (chosen by the
+    autotuner) regroups pencils into workgroups without changing any
+    pencil's arithmetic: a step with every variant forced to 1, and one after
+    autotuning, equal the default step bit for bit."""
+    import ics
+    shape = (64, 48, 64)
+    d = ics.params_dict("interface64", divv_switch=2)
+    d["nx"], d["ny"], d["nz"] = shape
+    qp = qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_})
+    psi = ics.workload_psi("interface64", shape=shape)
+    base = qsm_mod.Sim(qp, prec)
+    half = qsm_mod.Sim(qp, prec)
+    tuned = qsm_mod.Sim(qp, prec)
+    for ax in range(3):
+        for inv in range(2):
+            half.set_variant(ax, inv, 1)
+    tuned.autotune()
+    for s_ in (base, half, tuned):
+        s_.set_psi(psi)
+        s_.step(4)
+    for f in ("PSI", "PSIHAT", "RHO", "VX", "VY", "VZ", "NHAT"):
+        a = base.field(f)
+        assert np.array_equal(a, half.field(f)), f
+        assert np.array_equal(a, tuned.field(f)), f
