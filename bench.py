@@ -45,6 +45,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(symbol):
+    """HBM bytes per launch of `symbol` measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, each calibrated on a copy kernel of known size)
+    and recorded in profiles/traffic.json; None when that kernel was not
+    measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(symbol)
+    return None if k is None else float(k["bytes_per_launch"])
+
+
 def make_sim(qsm, ics, name, prec, shape):
     d = ics.params_dict(name)
     d["nx"], d["ny"], d["nz"] = shape
@@ -163,7 +178,7 @@ def main():
     k = top
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
     symbol = (f"void qsm::k_pass<float, {shape[0]}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
-              f"{k['ld']}, {k['st']}>(qsm::PassArgs)")
+              f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
     direction = "inverse" if k["inv"] else "forward"
     out = {
         "metric": "grid-pt updates/s at 512^3, mixed vs FP64 on 1 GPU; % HBM roofline; psi rel-L2",
@@ -182,7 +197,7 @@ def main():
                                "(FP32 fields/transforms, FP64 psihat, coefficients, mean modes)",
                    "grid": list(shape), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(symbol),
                      "kernel": symbol,
                      "pass": f"{direction} {'xyz'[k['axis']]} pass, load op {LOAD_OPS[k['ld']]}, "
                              f"store op {STORE_OPS[k['st']]}",

@@ -87,7 +87,7 @@ bool kind_inv(int k) { return k == QSM_REDFT01 || k == QSM_RODFT01; }
 bool kind_dst(int k) { return k == QSM_RODFT01 || k == QSM_RODFT10; }
 
 // ---- per-launch timing with HIP events.  Categories: one per kernel symbol
-// k_pass<T, N, axis, inv, LD, ST> (key = ((axis*2 + inv)*64 + LD)*16 + ST).
+// k_pass<T, N, axis, inv, LD, ST, V> (key = (((axis*2 + inv)*64 + LD)*16 + ST)*2 + V).
 struct Timer {
   bool on = false;
   struct Rec { hipEvent_t a, b; int cat; double bytes; };
@@ -122,7 +122,9 @@ struct Timer {
   }
 };
 
-int timer_key(int axis, bool inv, int ld, int st) { return ((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st; }
+int timer_key(int axis, bool inv, int ld, int st, int v) {
+  return (((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st) * 2 + v;
+}
 
 }  // namespace
 
@@ -349,7 +351,7 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch (axis %d, n %d): %s", axis, n, hipGetErrorString(e));
   if (T.on) {
     hipEventRecord(eb, h->stream);
-    T.recs.push_back({ea, eb, timer_key(axis, inv, L.code, S.code), op_bytes(h, L, S, first_pass)});
+    T.recs.push_back({ea, eb, timer_key(axis, inv, L.code, S.code, var), op_bytes(h, L, S, first_pass)});
   }
   return QSM_OK;
 }
@@ -895,7 +897,7 @@ extern "C" int qsm_timing_get(qsm_handle h, int which, double* total_ms, long lo
   double ms = 0, by = 0;
   long long c = 0;
   for (const auto& kv : h->timer.acc) {
-    const int axis = kv.first / (64 * 16) / 2, inv = (kv.first / (64 * 16)) % 2;
+    const int axis = kv.first / (64 * 16 * 2) / 2, inv = (kv.first / (64 * 16 * 2)) % 2;
     const int q = 1 + axis + (inv ? 0 : 3);
     if (which == 0 || which == q) { ms += kv.second.ms; by += kv.second.bytes; c += kv.second.count; }
   }
@@ -906,7 +908,8 @@ extern "C" int qsm_timing_get(qsm_handle h, int which, double* total_ms, long lo
 }
 
 extern "C" int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, int* ld, int* st,
-                                 double* total_ms, long long* launches, double* alg_bytes) {
+                                 int* variant, double* total_ms, long long* launches,
+                                 double* alg_bytes) {
   if (!h || rank < 0) return fail(QSM_ERR_ARG, "qsm_timing_kernel: bad argument");
   h->timer.drain();
   std::vector<std::pair<double, int>> order;
@@ -915,10 +918,11 @@ extern "C" int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, in
   if (rank >= (int)order.size()) return fail(QSM_ERR_ARG, "qsm_timing_kernel: only %zu kernels timed", order.size());
   const int key = order[rank].second;
   const auto& a = h->timer.acc[key];
-  if (axis) *axis = key / (64 * 16) / 2;
-  if (inv) *inv = (key / (64 * 16)) % 2;
-  if (ld) *ld = (key / 16) % 64;
-  if (st) *st = key % 16;
+  if (axis) *axis = key / (64 * 16 * 2) / 2;
+  if (inv) *inv = (key / (64 * 16 * 2)) % 2;
+  if (ld) *ld = (key / (16 * 2)) % 64;
+  if (st) *st = (key / 2) % 16;
+  if (variant) *variant = key % 2;
   if (total_ms) *total_ms = a.ms;
   if (launches) *launches = a.count;
   if (alg_bytes) *alg_bytes = a.bytes;

@@ -99,24 +99,66 @@ __device__ __forceinline__ double rd_if(const void* p, long idx, long off, bool 
   return ok ? v : 0.0;
 }
 
+// Wavenumber values at one element of an x pass.  An x-pass thread keeps one
+// (j, k) for all its elements, so the y and z entries are read once per
+// thread; xi is the launch's i-table at the element's i (itab() below).
+struct Kv {
+  double xi;
+  double qy, sy, qz, sz, fy, fz;
+};
+
+// The one i-indexed table a launch's operator reads (uniform per launch),
+// or null: q_x or s_x for the spectral prologues, q_x for the update and the
+// filter factors f_x for the density filter.
+template <int LD, int ST>
+__device__ __forceinline__ const double* itab(const LoadOp& L, const Waves& W) {
+  if constexpr (LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
+                LD == LD_VEL1 || LD == LD_VEL2) {
+    return W.qx;
+  } else if constexpr (LD == LD_VEL0 || LD == LD_GRAD || LD == LD_GRAD_D) {
+    return W.sx;
+  } else if constexpr (LD == LD_LAPGRAD) {
+    return L.axis == 0 ? W.sx : W.qx;
+  } else if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
+    return W.qx;
+  } else if constexpr (ST == ST_FILTER) {
+    return W.fx;
+  } else {
+    return nullptr;
+  }
+}
+
+template <int LD, int ST>
+constexpr bool uses_kv() {
+  return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
+         LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_GRAD ||
+         LD == LD_GRAD_D || LD == LD_LAPGRAD || ST == ST_UPDATE || ST == ST_UPDATE_STB ||
+         ST == ST_FILTER;
+}
+
+__device__ __forceinline__ Kv kv_thread(const Waves& W, int j, int k) {
+  return Kv{0.0, W.qy[j], W.sy[j], W.qz[k], W.sz[k], W.fy[j], W.fz[k]};
+}
+
 // Shifted read along axis c used by the gradient stencils: -s_c[m] src[m+1],
 // 0 at m = N_c - 1.  The axis is uniform for a launch, so the per-element
 // choices below are selects, not branches.
 template <typename S>
-__device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Waves& W, int i,
+__device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Kv& K, int i,
                                            int j, int k, long idx) {
   const long nyz = (long)G.ny * G.nz;
   const int c = L.axis;
   const int m = c == 0 ? i : (c == 1 ? j : k);
   const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
   const long o = c == 0 ? nyz : (c == 1 ? (long)G.nz : 1L);
-  const double* st = c == 0 ? W.sx : (c == 1 ? W.sy : W.sz);
+  const double st = c == 0 ? K.xi : (c == 1 ? K.sy : K.sz);
   const bool ok = m < nc - 1;
-  return -st[m] * rd_if<S>(L.a[0], idx, o, ok);
+  return -st * rd_if<S>(L.a[0], idx, o, ok);
 }
 
+// K carries the element's wavenumbers for the x-pass spectral operators.
 template <typename T, int LD>
-__device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Waves& W,
+__device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv& K,
                                        const Geo& G, int i, int j, int k, long idx) {
   if constexpr (LD == LD_PLAIN) {
     return ((const T*)L.a[0])[idx];
@@ -125,22 +167,22 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
   } else if constexpr (LD == LD_ZERO) {
     return (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
-    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
     const double fh = P.alpha * (P.q02 - mq2) * ((const double*)L.a[0])[idx];
     return (T)(LD == LD_COEF_F ? fh : -mq2 * fh);
   } else if constexpr (LD == LD_GRAD) {
-    return (T)grad_hat<T>(L, G, W, i, j, k, idx);
+    return (T)grad_hat<T>(L, G, K, i, j, k, idx);
   } else if constexpr (LD == LD_GRAD_D) {
-    return (T)grad_hat<double>(L, G, W, i, j, k, idx);
+    return (T)grad_hat<double>(L, G, K, i, j, k, idx);
   } else if constexpr (LD == LD_LAPGRAD) {
+    // i-table: s_x when c = 0, else q_x
     const int c = L.axis;
-    const double ax = c == 0 ? W.sx[i] : W.qx[i];
-    const double ay = c == 1 ? W.sy[j] : W.qy[j];
-    const double az = c == 2 ? W.sz[k] : W.qz[k];
-    const double m = sq(ax) + sq(ay) + sq(az);
-    return (T)(-m * grad_hat<double>(L, G, W, i, j, k, idx));
+    const double ay = c == 1 ? K.sy : K.qy;
+    const double az = c == 2 ? K.sz : K.qz;
+    const double m = sq(K.xi) + sq(ay) + sq(az);
+    return (T)(-m * grad_hat<double>(L, G, K, i, j, k, idx));
   } else if constexpr (LD == LD_SH) {
-    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
     const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
     return (T)(-alin * ((const double*)L.a[0])[idx]);
   } else if constexpr (LD == LD_RHOSTAR) {
@@ -161,31 +203,32 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
     const T rf = r * ((const T*)L.a[2])[idx];
     return gg * ((const T*)L.a[3])[idx] - rf * ((const T*)L.a[4])[idx];
   } else if constexpr (LD == LD_PRESSURE) {
-    // a: Sx, Sy, Sz (forward DST outputs)
+    // a: Sx, Sy, Sz (forward DST outputs); i-table q_x
     const long nyz = (long)G.ny * G.nz;
     const double a = rd_if<T>(L.a[0], idx, -nyz, i > 0);
     const double b = rd_if<T>(L.a[1], idx, -(long)G.nz, j > 0);
     const double c = rd_if<T>(L.a[2], idx, -1, k > 0);
-    const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
-    const double pv = P.scale * (-(W.qx[i] * a + W.qy[j] * b + W.qz[k] * c) / mq2);
+    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
+    const double pv = P.scale * (-(K.xi * a + K.qy * b + K.qz * c) / mq2);
     return (T)((i + j + k == 0) ? 0.0 : pv);
   } else if constexpr (LD == LD_VEL0) {
-    // a: Sx, Sy, Sz, Dhat (may be null).  Global-index form of quasi.cpp:1522-1634.
+    // a: Sx, Sy, Sz, Dhat (may be null); i-table s_x.  Global-index form of
+    // quasi.cpp:1522-1634.
     const long nyz = (long)G.ny * G.nz;
     const bool in = i < G.nx - 1;  // v_x = 0 at i = Nx-1
     const double nu = P.nu;
-    const double rm2 = 1.0 / (sq(W.sx[i]) + sq(W.qy[j]) + sq(W.qz[k]));
-    const double cm1 = (P.scale / nu) * rm2, cm2 = W.sx[i] * rm2;
+    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.qz));
+    const double cm1 = (P.scale / nu) * rm2, cm2 = K.xi * rm2;
     const double sx = rd<T>(L.a[0], idx);
     const double syv = rd_if<T>(L.a[1], idx, nyz - G.nz, in && j > 0);
     const double szv = rd_if<T>(L.a[2], idx, nyz - 1, in && k > 0);
-    const double dot = W.sx[i] * sx + W.qy[j] * syv + W.qz[k] * szv;
+    const double dot = K.xi * sx + K.qy * syv + K.qz * szv;
     const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, nyz, in) : 0.0;
-    const double vx = cm1 * (sx - cm2 * dot + nu * W.sx[i] * dd);
+    const double vx = cm1 * (sx - cm2 * dot + nu * K.xi * dd);
     return (T)(in ? vx : 0.0);
   } else if constexpr (LD == LD_VEL1 || LD == LD_VEL2) {
-    // quasi.cpp:1681-1872: zero at j = Ny-1 or k = Nz-1.  The S_x neighbour
-    // slab is i-1, or the P-rank halo slab (quasi.cpp:1639-1650).
+    // quasi.cpp:1681-1872: zero at j = Ny-1 or k = Nz-1; i-table q_x.  The
+    // S_x neighbour slab is i-1, or the P-rank halo slab (quasi.cpp:1639-1650).
     const long nyz = (long)G.ny * G.nz;
     const bool in = j < G.ny - 1 && k < G.nz - 1;
     const double nu = P.nu;
@@ -194,21 +237,21 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Wav
     const long dx = (long)(ih - i) * nyz;
     double vv;
     if constexpr (LD == LD_VEL1) {
-      const double rm2 = 1.0 / (sq(W.qx[i]) + sq(W.sy[j]) + sq(W.qz[k]));
+      const double rm2 = 1.0 / (sq(K.xi) + sq(K.sy) + sq(K.qz));
       const double sy = rd<T>(L.a[1], idx);
       const double sxa = rd_if<T>(L.a[0], idx, dx + G.nz, in && i > 0);
       const double szv = rd_if<T>(L.a[2], idx, G.nz - 1, in && k > 0);
-      const double dot = W.qx[i] * sxa + W.sy[j] * sy + W.qz[k] * szv;
+      const double dot = K.xi * sxa + K.sy * sy + K.qz * szv;
       const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, G.nz, in) : 0.0;
-      vv = ((P.scale / nu) * rm2) * (sy - (W.sy[j] * rm2) * dot + nu * W.sy[j] * dd);
+      vv = ((P.scale / nu) * rm2) * (sy - (K.sy * rm2) * dot + nu * K.sy * dd);
     } else {
-      const double rm2 = 1.0 / (sq(W.qx[i]) + sq(W.qy[j]) + sq(W.sz[k]));
+      const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.sz));
       const double sz = rd<T>(L.a[2], idx);
       const double sxb = rd_if<T>(L.a[0], idx, dx + 1, in && i > 0);
       const double syv = rd_if<T>(L.a[1], idx, 1 - (long)G.nz, in && j > 0);
-      const double dot = W.qx[i] * sxb + W.qy[j] * syv + W.sz[k] * sz;
+      const double dot = K.xi * sxb + K.qy * syv + K.sz * sz;
       const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, 1, in) : 0.0;
-      vv = ((P.scale / nu) * rm2) * (sz - (W.sz[k] * rm2) * dot + nu * W.sz[k] * dd);
+      vv = ((P.scale / nu) * rm2) * (sz - (K.sz * rm2) * dot + nu * K.sz * dd);
     }
     return (T)(in ? vv : 0.0);
   } else if constexpr (LD == LD_SCALE) {
@@ -266,15 +309,14 @@ constexpr bool store_reads() {
 }
 
 template <typename T, int ST>
-__device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Waves& W, int i, int j, int k,
-                                             long idx) {
+__device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Kv& K, long idx) {
   StIn in{0.0, 0.0, 0.0};
   if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
     in.ph = ((const double*)S.s[0])[idx];
     in.pho = ((const double*)S.d[0])[idx];
     in.nold = (double)((const T*)S.d[1])[idx];
   } else if constexpr (ST == ST_FILTER) {
-    in.ph = W.fx[i] * W.fy[j] * W.fz[k];
+    in.ph = K.xi * K.fy * K.fz;  // i-table f_x
   }
   return in;
 }
@@ -282,8 +324,8 @@ __device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Waves& W, i
 // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out);
 // s0: psihat (double, in).  quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
 template <typename T, int ST>
-__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Waves& W, int i,
-                                            int j, int k, long idx, T v, const StIn& in) {
+__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Kv& K, long idx,
+                                            T v, const StIn& in) {
   if constexpr (ST == ST_FILTER) {
     // quasi.cpp:1260-1261 with exp(-filter K^2) = fx[i] fy[j] fz[k] (in.ph)
     double raw = (double)v;
@@ -293,7 +335,7 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
     if (S.dc_out && idx == 0) S.dc_out[0] = r;
     return;
   }
-  const double mq2 = sq(W.qx[i]) + sq(W.qy[j]) + sq(W.qz[k]);
+  const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);  // i-table q_x
   const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
   double nh = (double)v;
   if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
@@ -323,23 +365,12 @@ constexpr bool load_heavy() {
   return LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_PRESSURE;
 }
 
+// Plain stores; the operators that read other fields go through
+// store_inputs/store_apply.
 template <typename T, int ST>
-__device__ __forceinline__ void store_eval(const StoreOp& S, const Phys& P, const Waves& W, int i,
-                                           int j, int k, long idx, T v) {
-  if constexpr (ST == ST_PLAIN) {
-    ((T*)S.d[0])[idx] = v;
-  } else if constexpr (ST == ST_FILTER) {
-    double raw = (double)v;
-    if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
-    const double r = P.scale * raw * (W.fx[i] * W.fy[j] * W.fz[k]);
-    ((T*)S.d[0])[idx] = (T)r;
-    if (S.dc_out && idx == 0) S.dc_out[0] = r;
-  } else if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
-    StIn in = store_inputs<T, ST>(S, idx);
-    store_apply<T, ST>(S, P, W, i, j, k, idx, v, in);
-  } else {
-    static_assert(ST == ST_PLAIN, "unknown store operator");
-  }
+__device__ __forceinline__ void store_eval(const StoreOp& S, long idx, T v) {
+  static_assert(ST == ST_PLAIN, "reading store operators use store_inputs/store_apply");
+  ((T*)S.d[0])[idx] = v;
 }
 
 }  // namespace qsm

@@ -111,9 +111,23 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   const int tid = threadIdx.x;
   const bool dst = A.dst != 0;
 
+  // The x-pass spectral operators read one i-indexed wavenumber table per
+  // launch (itab); it is staged in LDS where two workgroups per CU still fit
+  // (160 KiB), else read from global memory per element.
+  constexpr bool KV = uses_kv<LD, ST>();
+  static_assert(!KV || AX == kAxX, "wavenumber operators run in x passes");
+  constexpr bool XT_LDS =
+      KV && sizeof(cx<T>) * ((long)M * TKP + N + 1) + sizeof(double) * N <= 81920;
+  __shared__ double xt[XT_LDS ? N : 1];
+  const double* xsrc = nullptr;
+  if constexpr (KV) xsrc = itab<LD, ST>(A.ld, A.waves);
   {
     const cx<T>* __restrict__ tg = (const cx<T>*)A.tw;
     for (int u = tid; u <= N; u += BLOCK) tq[u] = tg[u];
+    if constexpr (XT_LDS) {
+      for (int u = tid; u < N; u += BLOCK) xt[u] = xsrc[u];
+      __syncthreads();
+    }
   }
 
   TileGeo<AX> tg{};
@@ -130,6 +144,21 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
 
   // FFT-phase thread coordinates: pencil p, thread t within the pencil
   const int p = tid % TK, t = tid / TK;
+  // wavenumbers of this thread's (j, k); an x-pass thread keeps its column
+  // p = tid % TK in every phase
+  Kv kv{};
+  if constexpr (KV) {
+    int i0, j0, k0;
+    long idx0;
+    tg.at(p, 0, i0, j0, k0, idx0);
+    kv = kv_thread(A.waves, j0, k0);
+  }
+  auto kv_at = [&](int i) {
+    Kv K = kv;
+    if constexpr (XT_LDS) K.xi = xt[i];
+    else if constexpr (KV) K.xi = xsrc[i];
+    return K;
+  };
   double dsum = 0.0;  // FP64 sum of the loaded values (mixed-precision DC modes)
   cx<T> v[E];
 
@@ -152,7 +181,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           int i, j, k;
           long idx;
           tg.at(q, n, i, j, k, idx);
-          xs[c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -200,7 +229,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
       } else {
         int i, j, k;
         long idx;
-        if (tg.at(p, n, i, j, k, idx)) store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, val);
+        if (tg.at(p, n, i, j, k, idx)) store_eval<T, ST>(A.st, idx, val);
       }
     };
     if constexpr (STAGED) __syncthreads();
@@ -231,7 +260,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           int i, j, k;
           long idx;
           tg.at(q, n, i, j, k, idx);
-          in[c] = store_inputs<T, ST>(A.st, A.waves, i, j, k, idx);
+          in[c] = store_inputs<T, ST>(A.st, kv_at(i), idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -242,8 +271,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           long idx;
           if (!tg.at(q, n, i, j, k, idx)) continue;
           const T y = rbuf[((n >> 1) * TKP + q) * 2 + (n & 1)];
-          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y, in[c]);
-          else store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y);
+          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i), idx, y, in[c]);
+          else store_eval<T, ST>(A.st, idx, y);
         }
       }
     }
@@ -263,7 +292,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           int i, j, k;
           long idx;
           tg.at(q, n, i, j, k, idx);
-          xs[c] = load_eval<T, LD>(A.ld, A.phys, A.waves, G, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -313,7 +342,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
         const int vq = vpos<N>(n);
         T y = rbuf[((vq >> 1) * TKP + q) * 2 + (vq & 1)];
         if (dst && (n & 1)) y = -y;
-        store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y);
+        store_eval<T, ST>(A.st, idx, y);
       }
     } else {
       run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, v, buf, p, t, tq);
@@ -334,7 +363,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
               T y = c ? z.y : z.x;
               if (dst && (n & 1)) y = -y;
               tg.at(p, n, i, j, k, idx);
-              store_eval<T, ST>(A.st, A.phys, A.waves, i, j, k, idx, y);
+              store_eval<T, ST>(A.st, idx, y);
             }
           }
         }

@@ -96,7 +96,7 @@ SIGNATURES = {
     "qsm_set_variant": ([_VP, _IP, _IP, _IP], _IP),
     "qsm_timing_enable": ([_VP, _IP], _IP),
     "qsm_timing_get": ([_VP, _IP, _DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
-    "qsm_timing_kernel": ([_VP, _IP] + [ctypes.POINTER(_IP)] * 4 +
+    "qsm_timing_kernel": ([_VP, _IP] + [ctypes.POINTER(_IP)] * 5 +
                           [_DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
 }
 
@@ -243,13 +243,13 @@ class Sim:
 
     def timing_kernel(self, rank=0):
         """rank-th kernel by device time: {axis, inv, ld, st, ms, launches, bytes}"""
-        ax, inv, ld, st = (ctypes.c_int() for _ in range(4))
+        ax, inv, ld, st, var = (ctypes.c_int() for _ in range(5))
         ms, cnt, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
         _check(lib().qsm_timing_kernel(self.h, rank, ctypes.byref(ax), ctypes.byref(inv),
-                                       ctypes.byref(ld), ctypes.byref(st), ctypes.byref(ms),
-                                       ctypes.byref(cnt), ctypes.byref(by)))
+                                       ctypes.byref(ld), ctypes.byref(st), ctypes.byref(var),
+                                       ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)))
         return {"axis": ax.value, "inv": inv.value, "ld": ld.value, "st": st.value,
-                "ms": ms.value, "launches": cnt.value, "bytes": by.value}
+                "variant": var.value, "ms": ms.value, "launches": cnt.value, "bytes": by.value}
 
 
 class Ensemble:
