@@ -46,7 +46,11 @@ int fail(int code, const char* fmt, ...) {
                   __FILE__, __LINE__);                                                  \
   } while (0)
 
-// ---- twiddle tables: e^{-2 pi i t/(4N)}, t < 4N, per (device, N, precision)
+// ---- twiddle tables per (device, N, precision), in the layout the pass
+// kernel stages (stage twiddles, DCT-III pre- and DCT-II post-processing
+// twiddles; qsm_fft.hpp).  Every entry is a quarter-turn value
+// e^{-2 pi i t/(4N)}, t < N, computed in long double and rounded once, moved
+// to its quadrant by exact sign flips and swaps.
 std::mutex g_tw_mu;
 std::map<std::tuple<int, int, int>, void*> g_tw;
 
@@ -57,25 +61,45 @@ const void* twiddles(int n, int fp32) {
   auto key = std::make_tuple(dev, n, fp32);
   auto it = g_tw.find(key);
   if (it != g_tw.end()) return it->second;
-  const int m = 4 * n;
-  std::vector<double> re(m), im(m);
-  for (int t = 0; t < m; ++t) {
-    const long double a = -2.0L * 3.141592653589793238462643383279502884L * t / m;
-    re[t] = (double)cosl(a);
-    im[t] = (double)sinl(a);
+  std::vector<double> qre(n), qim(n);
+  for (int t = 0; t < n; ++t) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * t / (4.0L * n);
+    qre[t] = (double)cosl(a);
+    qim[t] = (double)sinl(a);
+  }
+  // e^{-2 pi i u/(4N)} = (-i)^(u div N) e^{-2 pi i (u mod N)/(4N)}, u < 4N
+  auto twq = [&](int u, double& re, double& im) {
+    const int q = u / n, r = u - q * n;
+    const double x = qre[r], y = qim[r];
+    const double a = (q & 1) ? y : x, b = (q & 1) ? x : y;
+    re = (q & 2) ? -a : a;
+    im = ((q ^ (q >> 1)) & 1) ? -b : b;
+  };
+  const int m = n / 2;
+  std::vector<double> h;  // interleaved re, im
+  auto put = [&](double re, double im) { h.push_back(re); h.push_back(im); };
+  double re, im;
+  for (int v = 0; v < m; ++v) { twq(8 * v, re, im); put(re, im); }  // e^{-2 pi i v/M}
+  for (int k = 0; k < m; ++k) {                                      // conj twq(k), conj twq(4k)
+    twq(k, re, im); put(re, -im);
+    twq(4 * k, re, im); put(re, -im);
+  }
+  for (int k = 0; k <= m / 2; ++k) {                                 // twq(4k), twq(k), twq(M-k)
+    twq(4 * k, re, im); put(re, im);
+    twq(k, re, im); put(re, im);
+    twq(m - k, re, im); put(re, im);
   }
   void* d = nullptr;
   const size_t el = fp32 ? sizeof(float) : sizeof(double);
-  if (hipMalloc(&d, 2 * el * m) != hipSuccess) return nullptr;
+  if (hipMalloc(&d, el * h.size()) != hipSuccess) return nullptr;
+  hipError_t e;
   if (fp32) {
-    std::vector<float> h(2 * m);
-    for (int t = 0; t < m; ++t) { h[2 * t] = (float)re[t]; h[2 * t + 1] = (float)im[t]; }
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    std::vector<float> f(h.begin(), h.end());
+    e = hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice);
   } else {
-    std::vector<double> h(2 * m);
-    for (int t = 0; t < m; ++t) { h[2 * t] = re[t]; h[2 * t + 1] = im[t]; }
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
   }
+  if (e != hipSuccess) { hipFree(d); return nullptr; }
   g_tw[key] = d;
   return d;
 }
@@ -150,6 +174,9 @@ extern "C" int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, in
     return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: unknown r2r kind");
   if (prec != QSM_FP64 && prec != QSM_MIXED) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: bad precision");
   if (flags != 0) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: flags must be 0");
+  // kernels address a field as a base pointer plus a 32-bit byte offset
+  if ((long)nx * ny * nz * (prec == QSM_FP64 ? 8 : 4) > (1L << 32))
+    return fail(QSM_ERR_UNSUPPORTED, "qsm_r2r3d_plan: field larger than 4 GiB");
   const int n[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a)
     if (!pass_supported(n[a]))
@@ -335,6 +362,8 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   A.part = part;
   A.dst = dst ? 1 : 0;
   const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
+  if (part && (inv || axis != 2 || !dc_capable(L.code)))
+    return fail(QSM_ERR_STATE, "pass: FP64 mean-mode sum requested for a pass that has none");
   if (h->dry) return QSM_OK;
   const int var = (L.code == LD_PLAIN && S.code == ST_PLAIN) ? h->variant[axis][inv ? 1 : 0] : 0;
   if (h->rec) {
@@ -602,6 +631,10 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     return fail(QSM_ERR_ARG, "qsm_create: compat_ranks must divide nx");
   if (!(p.dt > 0) || !(p.nu > 0) || p.nw <= 0 || !(p.q0 > 0))
     return fail(QSM_ERR_ARG, "qsm_create: dt, nu, q0 and nw must be positive");
+  // psihat is FP64 in both precisions and kernels address a field as a base
+  // pointer plus a 32-bit byte offset: at most 2^29 points
+  if ((long)p.nx * p.ny * p.nz > (1L << 29))
+    return fail(QSM_ERR_UNSUPPORTED, "qsm_create: more than 2^29 grid points");
   const int nn[3] = {p.nx, p.ny, p.nz};
   for (int a = 0; a < 3; ++a)
     if (!step_supported(nn[a]))

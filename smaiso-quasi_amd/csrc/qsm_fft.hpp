@@ -149,20 +149,22 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 }
 
 // ---------------- twiddles ---------------------------------------------------
-// tq: LDS copy of e^{-2 pi i t/(4N)} for t in [0, N] (a quarter turn).  Any
-// e^{-2 pi i u/(4N)}, u in [0, 4N), is tq[u mod N] times (-i)^(u div N).
-template <typename T, int N>
-__device__ __forceinline__ cx<T> twq(const cx<T>* tq, int u) {
-  // (-i)^q (x + i y): q=0 (x, y), q=1 (y, -x), q=2 (-x, -y), q=3 (-y, x).
-  // Selects, not branches: the quadrant differs from lane to lane.
-  const int q = u / N, r = u - q * N;
-  const cx<T> w = tq[r];
-  const bool sw = (q & 1) != 0;
-  const T a = sw ? w.y : w.x, b = sw ? w.x : w.y;
-  const T sa = (q & 2) ? (T)-1 : (T)1;
-  const T sb = ((q ^ (q >> 1)) & 1) ? (T)-1 : (T)1;
-  return cx<T>{sa * a, sb * b};
-}
+// Per length N (M = N/2) the host builds one table of complex values, all
+// taken from the quarter turn e^{-2 pi i t/(4N)}, t < N, by exact symmetries
+// (twq(u) = e^{-2 pi i u/(4N)} below):
+//   [0, M)               stage twiddles    e^{-2 pi i v/M} = twq(8v)
+//   [M, 3M)              DCT-III pre-processing, per m < M:
+//                        conj twq(m), conj twq(4m)
+//   [3M, 3M + 3(M/2+1))  DCT-II post-processing, per k <= M/2:
+//                        twq(4k), twq(k), twq(M-k)
+// A pass stages the stage part and its direction's part in LDS; every
+// twiddle is then one LDS read at a per-thread base plus a constant.
+template <int M>
+struct TwLayout {
+  static constexpr int STAGE = 0, PRE = M, POST = 3 * M;
+  static constexpr int NPRE = 2 * M, NPOST = 3 * (M / 2 + 1);
+  static constexpr int TOTAL = 3 * M + NPOST;
+};
 
 // ---------------- Stockham stages ------------------------------------------
 // The M-point FFT of TK pencils lives in LDS as buf[m * TKP + p] (complex
@@ -215,7 +217,7 @@ __device__ __forceinline__ void scatter(const cx<T>* v, cx<T>* buf, int p, int t
 
 // twiddle + radix-R DFT on the thread's registers; S = -1 forward, +1 inverse
 template <typename T, int N, int M, int E, int R, int NS, int S>
-__device__ __forceinline__ void stage_compute(cx<T>* v, int t, const cx<T>* tq) {
+__device__ __forceinline__ void stage_compute(cx<T>* v, int t, const cx<T>* ts) {
   constexpr int TP = M / E, NB = E / R;
   static_assert(E % R == 0, "E must be a multiple of every radix");
 #pragma unroll
@@ -224,7 +226,7 @@ __device__ __forceinline__ void stage_compute(cx<T>* v, int t, const cx<T>* tq) 
       const int jm = (t + b * TP) % NS;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        cx<T> w = twq<T, N>(tq, 8 * (jm * r * (M / (NS * R))));  // e^{-2 pi i u/M}, 4N = 8M
+        cx<T> w = ts[jm * r * (M / (NS * R))];  // e^{-2 pi i jm r/(NS R)}
         if (S > 0) w.y = -w.y;
         v[b * R + r] = cmul(v[b * R + r], w);
       }
@@ -240,8 +242,8 @@ __device__ __forceinline__ void stage_compute(cx<T>* v, int t, const cx<T>* tq) 
 template <typename T, int N, int M, int E, int TKP, int S, bool LAST_TO_LDS, int NS, int R,
           int... RS>
 __device__ __forceinline__ void fft_from_regs(cx<T>* v, cx<T>* buf, int p, int t,
-                                              const cx<T>* tq) {
-  stage_compute<T, N, M, E, R, NS, S>(v, t, tq);
+                                              const cx<T>* ts) {
+  stage_compute<T, N, M, E, R, NS, S>(v, t, ts);
   if constexpr (sizeof...(RS) == 0) {
     if constexpr (LAST_TO_LDS) {
       __syncthreads();
@@ -253,14 +255,14 @@ __device__ __forceinline__ void fft_from_regs(cx<T>* v, cx<T>* buf, int p, int t
     scatter<T, M, E, TKP, R, NS>(v, buf, p, t);
     __syncthreads();
     gather<T, M, E, TKP, First<RS...>::value>(v, buf, p, t);
-    fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, NS * R, RS...>(v, buf, p, t, tq);
+    fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, NS * R, RS...>(v, buf, p, t, ts);
   }
 }
 
 template <typename T, int N, int M, int E, int TKP, int S, bool LAST_TO_LDS, int... RS>
 __device__ __forceinline__ void run_fft(Radices<RS...>, cx<T>* v, cx<T>* buf, int p, int t,
-                                        const cx<T>* tq) {
-  fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, 1, RS...>(v, buf, p, t, tq);
+                                        const cx<T>* ts) {
+  fft_from_regs<T, N, M, E, TKP, S, LAST_TO_LDS, 1, RS...>(v, buf, p, t, ts);
 }
 
 // Radix plan per half-length M: E elements per thread and the radix list.

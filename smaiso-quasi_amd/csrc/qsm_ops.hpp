@@ -85,17 +85,34 @@ struct StoreOp {
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// Element index into a field.  Fields hold at most 2^29 elements (checked
+// on the host), so the byte offset of any FP64 element fits in 32 bits and
+// every access is a uniform base pointer plus a 32-bit unsigned offset (the
+// scalar-base addressing form, one vector add per access).
+using Ix = unsigned;
+
 template <typename T>
-__device__ __forceinline__ double rd(const void* p, long idx) {
-  return (double)((const T*)p)[idx];
+__device__ __forceinline__ T ld(const void* base, Ix idx) {
+  const Ix off = idx * (Ix)sizeof(T);
+  return *(const T*)((const char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void st(void* base, Ix idx, T v) {
+  const Ix off = idx * (Ix)sizeof(T);
+  *(T*)((char*)base + off) = v;
+}
+
+template <typename T>
+__device__ __forceinline__ double rd(const void* p, Ix idx) {
+  return (double)ld<T>(p, idx);
 }
 
 // p[idx + off] if ok, else 0, without a branch: the load always reads a valid
 // address (idx itself when !ok) and the result is selected afterwards, so
 // the loads of one element can be issued together.
 template <typename T>
-__device__ __forceinline__ double rd_if(const void* p, long idx, long off, bool ok) {
-  const double v = (double)((const T*)p)[ok ? idx + off : idx];
+__device__ __forceinline__ double rd_if(const void* p, Ix idx, int off, bool ok) {
+  const double v = (double)ld<T>(p, ok ? idx + (Ix)off : idx);
   return ok ? v : 0.0;
 }
 
@@ -145,30 +162,138 @@ __device__ __forceinline__ Kv kv_thread(const Waves& W, int j, int k) {
 // choices below are selects, not branches.
 template <typename S>
 __device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Kv& K, int i,
-                                           int j, int k, long idx) {
-  const long nyz = (long)G.ny * G.nz;
+                                           int j, int k, Ix idx) {
+  const int nyz = G.ny * G.nz;
   const int c = L.axis;
   const int m = c == 0 ? i : (c == 1 ? j : k);
   const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
-  const long o = c == 0 ? nyz : (c == 1 ? (long)G.nz : 1L);
+  const int o = c == 0 ? nyz : (c == 1 ? G.nz : 1);
   const double st = c == 0 ? K.xi : (c == 1 ? K.sy : K.sz);
   const bool ok = m < nc - 1;
   return -st * rd_if<S>(L.a[0], idx, o, ok);
 }
 
+// The projection prologues (pressure and the three velocity components) read
+// shifted neighbours of three or four fields and do FP64 arithmetic with one
+// division.  They run in two phases: fetch issues every load of an element
+// (raw values at valid addresses, nothing else), finish selects and
+// evaluates.  A pass fetches a chunk of elements before finishing any, so
+// the chunk's loads are in flight together.
+template <int LD>
+constexpr bool load_split() {
+  return LD == LD_PRESSURE || LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2;
+}
+
+template <typename T>
+struct Raw4 {
+  T a, b, c, d;
+};
+
+// branch-free logical and (both operands are evaluated)
+__device__ __forceinline__ bool band(bool a, bool b) { return (int)a & (int)b; }
+
+// flags and offsets of the shifted reads, shared by fetch and finish
+struct Shifts {
+  bool fa, fb, fc, fd;   // read a/b/c/d is live (else its value is 0)
+  int oa, ob, oc, od;    // index offsets of the reads
+};
+
+template <int LD>
+__device__ __forceinline__ Shifts shifts(const LoadOp& L, const Geo& G, int i, int j, int k) {
+  const int nyz = G.ny * G.nz;
+  const bool hasd = L.a[3] != nullptr;  // D-hat is null while the divergence term is off
+  if constexpr (LD == LD_PRESSURE) {
+    // a: S_x[i-1], b: S_y[j-1], c: S_z[k-1]
+    return Shifts{i > 0, j > 0, k > 0, false, -nyz, -G.nz, -1, 0};
+  } else if constexpr (LD == LD_VEL0) {
+    // a: S_x, b: S_y[i+1][j-1], c: S_z[i+1][k-1], d: D[i+1]; zero at i = Nx-1
+    const bool in = i < G.nx - 1;
+    return Shifts{true, band(in, j > 0), band(in, k > 0), band(in, hasd), 0, nyz - G.nz, nyz - 1, nyz};
+  } else {
+    // zero at j = Ny-1 or k = Nz-1.  The S_x neighbour slab is i-1, or the
+    // P-rank halo slab (quasi.cpp:1639-1650).
+    const bool in = (j < G.ny - 1) & (k < G.nz - 1);
+    // i mod n0 without an integer division: floor((i + 1/2)/n0) in FP32 is
+    // exact for i < 2^22 (relative error 2^-23 against a margin of 1/(2 n0))
+    const int n0 = L.n0 > 0 ? L.n0 : 1;
+    const int qn = (int)(((float)i + 0.5f) * (1.0f / (float)n0));
+    const bool halo = (L.n0 > 0) & (i > 0) & (i == qn * n0);
+    const int ih = halo ? i - L.n0 : i - 1;
+    const int dx = (ih - i) * nyz;
+    if constexpr (LD == LD_VEL1)  // a: S_y, b: S_x[ih][j+1], c: S_z[j+1][k-1], d: D[j+1]
+      return Shifts{true, band(in, i > 0), band(in, k > 0), band(in, hasd), 0, dx + G.nz, G.nz - 1, G.nz};
+    else  // a: S_z, b: S_x[ih][k+1], c: S_y[j-1][k+1], d: D[k+1]
+      return Shifts{true, band(in, i > 0), band(in, j > 0), band(in, hasd), 0, dx + 1, 1 - G.nz, 1};
+  }
+}
+
+template <typename T, int LD>
+__device__ __forceinline__ Raw4<T> fetch(const LoadOp& L, const Geo& G, int i, int j, int k,
+                                         Ix idx) {
+  const Shifts h = shifts<LD>(L, G, i, j, k);
+  // field of each read: PRESSURE S_x, S_y, S_z; VEL_c own component first
+  const void* pa = LD == LD_VEL1 ? L.a[1] : (LD == LD_VEL2 ? L.a[2] : L.a[0]);
+  const void* pb = (LD == LD_PRESSURE || LD == LD_VEL0) ? L.a[1] : L.a[0];
+  const void* pc = LD == LD_VEL2 ? L.a[1] : L.a[2];
+  const void* pd = L.a[3] ? L.a[3] : L.a[0];  // a valid address when D-hat is null
+  Raw4<T> r;
+  r.a = ld<T>(pa, h.fa ? idx + (Ix)h.oa : idx);
+  r.b = ld<T>(pb, h.fb ? idx + (Ix)h.ob : idx);
+  r.c = ld<T>(pc, h.fc ? idx + (Ix)h.oc : idx);
+  if constexpr (LD != LD_PRESSURE) r.d = ld<T>(pd, h.fd ? idx + (Ix)h.od : idx);
+  else r.d = (T)0;
+  return r;
+}
+
+template <typename T, int LD>
+__device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phys& P, const Kv& K,
+                                    const Geo& G, int i, int j, int k) {
+  const Shifts h = shifts<LD>(L, G, i, j, k);
+  const double a = h.fa ? (double)r.a : 0.0, b = h.fb ? (double)r.b : 0.0,
+               c = h.fc ? (double)r.c : 0.0, dd = h.fd ? (double)r.d : 0.0;
+  const double nu = P.nu;
+  if constexpr (LD == LD_PRESSURE) {
+    // quasi.cpp:1385-1415; i-table q_x
+    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
+    const double pv = P.scale * (-(K.xi * a + K.qy * b + K.qz * c) / mq2);
+    return (T)((i + j + k == 0) ? 0.0 : pv);
+  } else if constexpr (LD == LD_VEL0) {
+    // global-index form of quasi.cpp:1522-1634; i-table s_x
+    const bool in = i < G.nx - 1;
+    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.qz));
+    const double cm1 = (P.scale / nu) * rm2, cm2 = K.xi * rm2;
+    const double dot = K.xi * a + K.qy * b + K.qz * c;
+    const double vx = cm1 * (a - cm2 * dot + nu * K.xi * dd);
+    return (T)(in ? vx : 0.0);
+  } else if constexpr (LD == LD_VEL1) {
+    // quasi.cpp:1681-1872; i-table q_x
+    const bool in = (j < G.ny - 1) & (k < G.nz - 1);
+    const double rm2 = 1.0 / (sq(K.xi) + sq(K.sy) + sq(K.qz));
+    const double dot = K.xi * b + K.sy * a + K.qz * c;
+    const double vv = ((P.scale / nu) * rm2) * (a - (K.sy * rm2) * dot + nu * K.sy * dd);
+    return (T)(in ? vv : 0.0);
+  } else {
+    const bool in = (j < G.ny - 1) & (k < G.nz - 1);
+    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.sz));
+    const double dot = K.xi * b + K.qy * c + K.sz * a;
+    const double vv = ((P.scale / nu) * rm2) * (a - (K.sz * rm2) * dot + nu * K.sz * dd);
+    return (T)(in ? vv : 0.0);
+  }
+}
+
 // K carries the element's wavenumbers for the x-pass spectral operators.
 template <typename T, int LD>
 __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv& K,
-                                       const Geo& G, int i, int j, int k, long idx) {
+                                       const Geo& G, int i, int j, int k, Ix idx) {
   if constexpr (LD == LD_PLAIN) {
-    return ((const T*)L.a[0])[idx];
+    return ld<T>(L.a[0], idx);
   } else if constexpr (LD == LD_PLAIN_D) {
-    return (T)((const double*)L.a[0])[idx];
+    return (T)ld<double>(L.a[0], idx);
   } else if constexpr (LD == LD_ZERO) {
     return (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
     const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
-    const double fh = P.alpha * (P.q02 - mq2) * ((const double*)L.a[0])[idx];
+    const double fh = P.alpha * (P.q02 - mq2) * ld<double>(L.a[0], idx);
     return (T)(LD == LD_COEF_F ? fh : -mq2 * fh);
   } else if constexpr (LD == LD_GRAD) {
     return (T)grad_hat<T>(L, G, K, i, j, k, idx);
@@ -184,109 +309,59 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
   } else if constexpr (LD == LD_SH) {
     const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
     const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-    return (T)(-alin * ((const double*)L.a[0])[idx]);
+    return (T)(-alin * ld<double>(L.a[0], idx));
   } else if constexpr (LD == LD_RHOSTAR) {
-    const T ps = ((const T*)L.a[0])[idx], gx = ((const T*)L.a[1])[idx],
-            gy = ((const T*)L.a[2])[idx], gz = ((const T*)L.a[3])[idx];
+    const T ps = ld<T>(L.a[0], idx), gx = ld<T>(L.a[1], idx),
+            gy = ld<T>(L.a[2], idx), gz = ld<T>(L.a[3], idx);
     return (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
   } else if constexpr (LD == LD_DIVV) {
     // a: rho, rho_old, vx, vy, vz, rx, ry, rz
-    const T r = ((const T*)L.a[0])[idx], ro = ((const T*)L.a[1])[idx];
-    T t = (r - ro) / (T)P.dt + ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
-    t = t + ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
-    t = t + ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
+    const T r = ld<T>(L.a[0], idx), ro = ld<T>(L.a[1], idx);
+    T t = (r - ro) / (T)P.dt + ld<T>(L.a[2], idx) * ld<T>(L.a[5], idx);
+    t = t + ld<T>(L.a[3], idx) * ld<T>(L.a[6], idx);
+    t = t + ld<T>(L.a[4], idx) * ld<T>(L.a[7], idx);
     return -t / ((T)0.5 * (r + ro));
   } else if constexpr (LD == LD_STRESS) {
     // a: rho, G, F, g_c, l_c
-    const T r = ((const T*)L.a[0])[idx];
-    const T gg = r * ((const T*)L.a[1])[idx];
-    const T rf = r * ((const T*)L.a[2])[idx];
-    return gg * ((const T*)L.a[3])[idx] - rf * ((const T*)L.a[4])[idx];
-  } else if constexpr (LD == LD_PRESSURE) {
-    // a: Sx, Sy, Sz (forward DST outputs); i-table q_x
-    const long nyz = (long)G.ny * G.nz;
-    const double a = rd_if<T>(L.a[0], idx, -nyz, i > 0);
-    const double b = rd_if<T>(L.a[1], idx, -(long)G.nz, j > 0);
-    const double c = rd_if<T>(L.a[2], idx, -1, k > 0);
-    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
-    const double pv = P.scale * (-(K.xi * a + K.qy * b + K.qz * c) / mq2);
-    return (T)((i + j + k == 0) ? 0.0 : pv);
-  } else if constexpr (LD == LD_VEL0) {
-    // a: Sx, Sy, Sz, Dhat (may be null); i-table s_x.  Global-index form of
-    // quasi.cpp:1522-1634.
-    const long nyz = (long)G.ny * G.nz;
-    const bool in = i < G.nx - 1;  // v_x = 0 at i = Nx-1
-    const double nu = P.nu;
-    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.qz));
-    const double cm1 = (P.scale / nu) * rm2, cm2 = K.xi * rm2;
-    const double sx = rd<T>(L.a[0], idx);
-    const double syv = rd_if<T>(L.a[1], idx, nyz - G.nz, in && j > 0);
-    const double szv = rd_if<T>(L.a[2], idx, nyz - 1, in && k > 0);
-    const double dot = K.xi * sx + K.qy * syv + K.qz * szv;
-    const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, nyz, in) : 0.0;
-    const double vx = cm1 * (sx - cm2 * dot + nu * K.xi * dd);
-    return (T)(in ? vx : 0.0);
-  } else if constexpr (LD == LD_VEL1 || LD == LD_VEL2) {
-    // quasi.cpp:1681-1872: zero at j = Ny-1 or k = Nz-1; i-table q_x.  The
-    // S_x neighbour slab is i-1, or the P-rank halo slab (quasi.cpp:1639-1650).
-    const long nyz = (long)G.ny * G.nz;
-    const bool in = j < G.ny - 1 && k < G.nz - 1;
-    const double nu = P.nu;
-    int ih = i - 1;
-    if (L.n0 > 0 && i > 0 && (i % L.n0) == 0) ih = i - L.n0;
-    const long dx = (long)(ih - i) * nyz;
-    double vv;
-    if constexpr (LD == LD_VEL1) {
-      const double rm2 = 1.0 / (sq(K.xi) + sq(K.sy) + sq(K.qz));
-      const double sy = rd<T>(L.a[1], idx);
-      const double sxa = rd_if<T>(L.a[0], idx, dx + G.nz, in && i > 0);
-      const double szv = rd_if<T>(L.a[2], idx, G.nz - 1, in && k > 0);
-      const double dot = K.xi * sxa + K.sy * sy + K.qz * szv;
-      const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, G.nz, in) : 0.0;
-      vv = ((P.scale / nu) * rm2) * (sy - (K.sy * rm2) * dot + nu * K.sy * dd);
-    } else {
-      const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.sz));
-      const double sz = rd<T>(L.a[2], idx);
-      const double sxb = rd_if<T>(L.a[0], idx, dx + 1, in && i > 0);
-      const double syv = rd_if<T>(L.a[1], idx, 1 - (long)G.nz, in && j > 0);
-      const double dot = K.xi * sxb + K.qy * syv + K.sz * sz;
-      const double dd = L.a[3] ? rd_if<T>(L.a[3], idx, 1, in) : 0.0;
-      vv = ((P.scale / nu) * rm2) * (sz - (K.sz * rm2) * dot + nu * K.sz * dd);
-    }
-    return (T)(in ? vv : 0.0);
+    const T r = ld<T>(L.a[0], idx);
+    const T gg = r * ld<T>(L.a[1], idx);
+    const T rf = r * ld<T>(L.a[2], idx);
+    return gg * ld<T>(L.a[3], idx) - rf * ld<T>(L.a[4], idx);
+  } else if constexpr (LD == LD_PRESSURE || LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2) {
+    return finish<T, LD>(fetch<T, LD>(L, G, i, j, k, idx), L, P, K, G, i, j, k);
   } else if constexpr (LD == LD_SCALE) {
     return (T)(P.scale * rd<T>(L.a[0], idx));
   } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P) {
     // a: rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
-    const T r = ((const T*)L.a[0])[idx], ps = ((const T*)L.a[1])[idx];
+    const T r = ld<T>(L.a[0], idx), ps = ld<T>(L.a[1], idx);
     const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    T v = r * ((T)P.eps * ps - ((const T*)L.a[2])[idx] + (T)P.beta * ps3 - (T)P.gamma * ps5) -
-          r * ((const T*)L.a[3])[idx];
-    v = v - ((const T*)L.a[4])[idx] * ((const T*)L.a[7])[idx];
-    v = v - ((const T*)L.a[5])[idx] * ((const T*)L.a[8])[idx];
-    v = v - ((const T*)L.a[6])[idx] * ((const T*)L.a[9])[idx];
+    T v = r * ((T)P.eps * ps - ld<T>(L.a[2], idx) + (T)P.beta * ps3 - (T)P.gamma * ps5) -
+          r * ld<T>(L.a[3], idx);
+    v = v - ld<T>(L.a[4], idx) * ld<T>(L.a[7], idx);
+    v = v - ld<T>(L.a[5], idx) * ld<T>(L.a[8], idx);
+    v = v - ld<T>(L.a[6], idx) * ld<T>(L.a[9], idx);
     if constexpr (LD == LD_NONLIN_P) {
-      const T rx = ((const T*)L.a[11])[idx], ry = ((const T*)L.a[12])[idx],
-              rz = ((const T*)L.a[13])[idx];
-      v = v + ((const T*)L.a[10])[idx] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
+      const T rx = ld<T>(L.a[11], idx), ry = ld<T>(L.a[12], idx),
+              rz = ld<T>(L.a[13], idx);
+      v = v + ld<T>(L.a[10], idx) * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
                   (T)P.rho_s2;
     }
     return v;
   } else if constexpr (LD == LD_STRESS_STB) {
     // a: sh, psi, g_c ; rho = 1
-    const T ps = ((const T*)L.a[1])[idx];
+    const T ps = ld<T>(L.a[1], idx);
     const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    const T mu = ((const T*)L.a[0])[idx] - (T)P.beta * ps3 + (T)P.gamma * ps5;
-    return mu * ((const T*)L.a[2])[idx];
+    const T mu = ld<T>(L.a[0], idx) - (T)P.beta * ps3 + (T)P.gamma * ps5;
+    return mu * ld<T>(L.a[2], idx);
   } else if constexpr (LD == LD_NONLIN_STB || LD == LD_NONLIN_STB0) {
     // a: psi, vx, vy, vz, gx, gy, gz
-    const T ps = ((const T*)L.a[0])[idx];
+    const T ps = ld<T>(L.a[0], idx);
     const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
     T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
     if constexpr (LD == LD_NONLIN_STB) {
-      v = v - ((const T*)L.a[1])[idx] * ((const T*)L.a[4])[idx];
-      v = v - ((const T*)L.a[2])[idx] * ((const T*)L.a[5])[idx];
-      v = v - ((const T*)L.a[3])[idx] * ((const T*)L.a[6])[idx];
+      v = v - ld<T>(L.a[1], idx) * ld<T>(L.a[4], idx);
+      v = v - ld<T>(L.a[2], idx) * ld<T>(L.a[5], idx);
+      v = v - ld<T>(L.a[3], idx) * ld<T>(L.a[6], idx);
     }
     return v;
   } else {
@@ -309,39 +384,47 @@ constexpr bool store_reads() {
 }
 
 template <typename T, int ST>
-__device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Kv& K, long idx) {
+__device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Kv& K, Ix idx) {
   StIn in{0.0, 0.0, 0.0};
   if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
-    in.ph = ((const double*)S.s[0])[idx];
-    in.pho = ((const double*)S.d[0])[idx];
-    in.nold = (double)((const T*)S.d[1])[idx];
+    in.ph = ld<double>(S.s[0], idx);
+    in.pho = ld<double>(S.d[0], idx);
+    in.nold = (double)ld<T>(S.d[1], idx);
   } else if constexpr (ST == ST_FILTER) {
     in.ph = K.xi * K.fy * K.fz;  // i-table f_x
   }
   return in;
 }
 
+// Mixed precision: the FP64 (0,0,0) values a store operator substitutes at
+// index 0, read once per thread (uniform loads) instead of per element.
+struct DcIn {
+  double dc, old;
+  bool has_dc, has_old;
+};
+__device__ __forceinline__ DcIn dc_in(const StoreOp& S) {
+  return DcIn{S.dc ? S.dc[0] : 0.0, S.dc_old ? S.dc_old[0] : 0.0, S.dc != nullptr,
+              S.dc_old != nullptr};
+}
+
 // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out);
 // s0: psihat (double, in).  quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
 template <typename T, int ST>
-__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Kv& K, long idx,
-                                            T v, const StIn& in) {
+__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Kv& K, Ix idx,
+                                            T v, const StIn& in, const DcIn& D) {
+  const bool z = idx == 0;
   if constexpr (ST == ST_FILTER) {
     // quasi.cpp:1260-1261 with exp(-filter K^2) = fx[i] fy[j] fz[k] (in.ph)
-    double raw = (double)v;
-    if (S.dc && idx == 0) raw = 8.0 * S.dc[0];
+    const double raw = (D.has_dc & z) ? 8.0 * D.dc : (double)v;
     const double r = P.scale * raw * in.ph;
-    ((T*)S.d[0])[idx] = (T)r;
-    if (S.dc_out && idx == 0) S.dc_out[0] = r;
+    st<T>(S.d[0], idx, (T)r);
+    if (z && S.dc_out) S.dc_out[0] = r;
     return;
   }
   const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);  // i-table q_x
   const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-  double nh = (double)v;
-  if (S.dc && idx == 0) nh = 8.0 * S.dc[0];
-  double nold = in.nold;
-  if (S.first) nold = nh;
-  else if (S.dc_old && idx == 0) nold = S.dc_old[0];
+  const double nh = (D.has_dc & z) ? 8.0 * D.dc : (double)v;
+  const double nold = S.first ? nh : ((D.has_old & z) ? D.old : in.nold);
   double nw;
   if constexpr (ST == ST_UPDATE) {
     const double c1 = 2.0 - P.dt * P.rho_m * alin, c2 = 1.5 - P.dt * P.rho_m * alin;
@@ -350,9 +433,17 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
     const double c1 = 1.0 + (P.dt / 2) * alin, c2 = 1.0 - (P.dt / 2) * alin;
     nw = (c1 * in.ph + (P.dt / 2) * P.scale * (3.0 * nh - nold)) / c2;
   }
-  ((double*)S.d[0])[idx] = nw;
-  ((T*)S.d[1])[idx] = (T)nh;
-  if (S.dc_out && idx == 0) S.dc_out[0] = nh;
+  st<double>(S.d[0], idx, nw);
+  st<T>(S.d[1], idx, (T)nh);
+  if (z && S.dc_out) S.dc_out[0] = nh;
+}
+
+// Load operators whose forward transform keeps its (0,0,0) mode in FP64 in
+// mixed precision (the density law and the nonlinear terms): their forward
+// z pass accumulates an FP64 sum of the loaded values per workgroup.
+__host__ __device__ constexpr bool dc_capable(int ld) {
+  return ld == LD_RHOSTAR || ld == LD_NONLIN || ld == LD_NONLIN_P || ld == LD_NONLIN_STB ||
+         ld == LD_NONLIN_STB0;
 }
 
 // Load operators that read many fields or do FP64 projection arithmetic are
@@ -368,9 +459,9 @@ constexpr bool load_heavy() {
 // Plain stores; the operators that read other fields go through
 // store_inputs/store_apply.
 template <typename T, int ST>
-__device__ __forceinline__ void store_eval(const StoreOp& S, long idx, T v) {
+__device__ __forceinline__ void store_eval(const StoreOp& S, Ix idx, T v) {
   static_assert(ST == ST_PLAIN, "reading store operators use store_inputs/store_apply");
-  ((T*)S.d[0])[idx] = v;
+  st<T>(S.d[0], idx, v);
 }
 
 }  // namespace qsm

@@ -69,31 +69,166 @@ __device__ __forceinline__ int xpos(int q) {
   return q < N / 2 ? 2 * q : 2 * (N - 1 - q) + 1;
 }
 
-// coordinates of element n of pencil p of the tile.  Fields have fewer than
-// 2^31 points (checked on the host), so element indices are 32-bit.
-// at() clamps an out-of-range pencil to the tile's first one, so every
-// address it returns is inside the field; the caller discards such values.
-template <int AX>
-struct TileGeo {
-  int row0;
-  int t_i, t_j, k0;
-  int nrows;
-  int nx, ny, nz;
-  __device__ __forceinline__ bool at(int p, int n, int& i, int& j, int& k, long& idx) const {
-    if constexpr (AX == kAxZ) {
-      const int row = row0 + p;
-      const bool ok = row < nrows;
-      const int rr = ok ? row : row0;
-      i = rr / ny; j = rr - i * ny; k = n;
-      idx = (long)(rr * nz + n);
-      return ok;
+// Real sample n of pencil q sits in LDS at rbuf[lpos(n, q)] (complex slot
+// n/2 of the interleaved buf[slot * TKP + q]).
+template <int TKP>
+__device__ __forceinline__ int lpos(int n, int q) {
+  return ((n >> 1) * TKP + q) * 2 + (n & 1);
+}
+
+// LDS positions X(t + K) and X(K - t) of a thread's own pencil p for
+// compile-time K, from four per-thread bases: the parity of n and the half
+// index n/2 follow from the parities of t and K, so each access is a base
+// plus a constant.
+template <int TKP>
+struct PencilPos {
+  int ue, uo, de, dd;
+  __device__ __forceinline__ void init(int p, int t) {
+    ue = lpos<TKP>(t, p);                                   // t + K, K even
+    uo = (((t + 1) >> 1) * TKP + p) * 2 + 1 - (t & 1);      // t + K, K odd
+    de = (p - ((t + 1) >> 1) * TKP) * 2 + (t & 1);          // K - t, K even
+    dd = (p - (t >> 1) * TKP) * 2 + 1 - (t & 1);            // K - t, K odd
+  }
+  __device__ __forceinline__ int up(int K) const { return (K & 1) ? uo + (K - 1) * TKP : ue + K * TKP; }
+  __device__ __forceinline__ int dn(int K) const { return (K & 1) ? dd + (K - 1) * TKP : de + K * TKP; }
+};
+
+// LDS position of element s of a thread's row-sweeping walk (the order of
+// TileGeo::walk) for sample n at: n itself (MODE 0), its Makhoul position
+// vpos(n) (MODE 1) or the reversed position N-1-n (MODE 2), with the parity
+// of n for the DST sign.  Where the tile shape allows, pos(s) = a + (s mod R) b
+// + (s div R) c with per-thread a, b and constant c; otherwise it is computed
+// per element.
+template <int AX, int N, int TK, int TKP, int BLOCK, int MODE>
+struct WalkPos {
+  static constexpr int TP = BLOCK / TK;
+  // 1: strided axis, n = t + s TP; 2: z axis, N a multiple of BLOCK;
+  // 3: z axis, BLOCK a multiple of N; 0: general
+  static constexpr int KIND =
+      AX != 2 ? ((TP % (MODE == 1 ? 4 : 2) == 0) ? 1 : 0)
+              : ((N % BLOCK == 0 && BLOCK % 4 == 0) ? 2 : (BLOCK % N == 0 ? 3 : 0));
+  static constexpr int R = KIND == 2 ? N / BLOCK : 1;
+  int a, b;
+  bool odd;
+  __device__ __forceinline__ static int place(int n) {
+    return MODE == 1 ? vpos<N>(n) : (MODE == 2 ? N - 1 - n : n);
+  }
+  __device__ __forceinline__ void init(int tid) {
+    // n0: the thread's sample at s = 0; dn: the change of n per step of s
+    // (strided) or per BLOCK (z); q0: its pencil
+    int n0, q0, dn;
+    if constexpr (KIND == 1) { n0 = tid / TK; q0 = tid % TK; dn = TP; }
+    else if constexpr (KIND == 2) { n0 = tid; q0 = 0; dn = BLOCK; }
+    else { n0 = tid % N; q0 = tid / N; dn = 0; }
+    odd = (n0 & 1) != 0;
+    const int v0 = place(n0);
+    a = lpos<TKP>(v0, q0);
+    // place() moves by +dn (MODE 0), -dn (MODE 2) or +-dn/2 (MODE 1, by the
+    // parity of n0); the moves are even, so the half index moves by half that
+    const int mv = MODE == 0 ? dn : (MODE == 2 ? -dn : (odd ? -dn / 2 : dn / 2));
+    b = (mv / 2) * TKP * 2;
+  }
+  __device__ __forceinline__ int pos(int tid, int s, bool& oddn) const {
+    if constexpr (KIND == 1) {
+      oddn = odd;
+      return a + s * b;
+    } else if constexpr (KIND == 2) {
+      oddn = odd;
+      return a + (s % R) * b + (s / R) * 2;
+    } else if constexpr (KIND == 3) {
+      oddn = odd;
+      return a + s * (BLOCK / N) * 2;
     } else {
-      const bool ok = k0 + p < nz;
-      k = ok ? k0 + p : k0;
-      if constexpr (AX == kAxY) { i = t_i; j = n; } else { i = n; j = t_j; }
-      idx = (long)((i * ny + j) * nz + k);
-      return ok;
+      int q, n;
+      if constexpr (AX == 2) { q = (tid + s * BLOCK) / N; n = (tid + s * BLOCK) % N; }
+      else { q = tid % TK; n = tid / TK + s * TP; }
+      oddn = (n & 1) != 0;
+      return lpos<TKP>(place(n), q);
     }
+  }
+};
+
+// Element coordinates of one workgroup's tile.  Fields have at most 2^29
+// elements (checked on the host), so indices are 32-bit and every address is
+// a uniform base plus a 32-bit offset.  Out-of-range elements (a partial
+// last tile) get a valid index inside the field and ok = false; the caller
+// discards their values and skips their stores.
+//
+// Two enumerations:
+//   walk(s): element s of the thread's row-sweeping walk e = tid + s BLOCK
+//            (loads, staged stores): z passes read rows, strided passes read
+//            TK-column segments of successive rows; the index advances by a
+//            uniform step per s.
+//   at(n):   element n of this thread's pencil (strided passes; the FFT
+//            phase keeps pencil p = tid % TK).
+template <int AX, int N, int TK, int BLOCK>
+struct TileGeo {
+  int ny, nrows;
+  int row0;        // z: first row (i ny + j) of the tile
+  int t_i, t_j;    // strided: the tile's fixed i (y pass) or j (x pass)
+  int kc;          // strided: the thread's column, clamped into the field
+  bool colok;      // strided: the thread's column is inside the field
+  Ix cbase;        // strided: index of the thread's column at n = 0
+  Ix axs;          // strided: index stride along the pass axis
+  Ix wbase, wstep; // walk: index of element 0 and the uniform step per s
+
+  __device__ __forceinline__ void init(const Geo& G, int tile, int tid) {
+    ny = G.ny;
+    nrows = G.nx * G.ny;
+    if constexpr (AX == kAxZ) {
+      row0 = tile * TK;
+      wbase = (Ix)row0 * N + (Ix)tid;
+      wstep = BLOCK;
+    } else {
+      const int tiles_k = (G.nz + TK - 1) / TK;
+      const int a = tile / tiles_k;
+      const int k0 = (tile - a * tiles_k) * TK;
+      const int q = tid % TK;
+      colok = k0 + q < G.nz;
+      kc = colok ? k0 + q : k0;
+      if constexpr (AX == kAxY) {
+        t_i = a;
+        cbase = (Ix)a * G.ny * G.nz + (Ix)kc;
+        axs = G.nz;
+      } else {
+        t_j = a;
+        cbase = (Ix)a * G.nz + (Ix)kc;
+        axs = (Ix)G.ny * G.nz;
+      }
+      wbase = cbase + (Ix)(tid / TK) * axs;
+      wstep = (Ix)(BLOCK / TK) * axs;
+    }
+  }
+
+  __device__ __forceinline__ void walk(int tid, int s, int& q, int& n, int& i, int& j, int& k,
+                                       Ix& idx, bool& ok) const {
+    if constexpr (AX == kAxZ) {
+      const int e = tid + s * BLOCK;
+      q = e / N;
+      n = e % N;
+      const int row = row0 + q;
+      ok = row < nrows;
+      idx = ok ? wbase + (Ix)s * wstep : (Ix)n;
+      const int rr = ok ? row : 0;
+      i = rr / ny;
+      j = rr - i * ny;
+      k = n;
+    } else {
+      q = tid % TK;
+      n = tid / TK + s * (BLOCK / TK);
+      ok = colok;
+      idx = wbase + (Ix)s * wstep;
+      if constexpr (AX == kAxY) { i = t_i; j = n; } else { i = n; j = t_j; }
+      k = kc;
+    }
+  }
+
+  // strided passes only: element n of this thread's column
+  __device__ __forceinline__ void at(int n, int& i, int& j, int& k, Ix& idx) const {
+    static_assert(AX != kAxZ, "at() is for strided passes");
+    idx = cbase + (Ix)n * axs;
+    if constexpr (AX == kAxY) { i = t_i; j = n; } else { i = n; j = t_j; }
+    k = kc;
   }
 };
 
@@ -104,8 +239,11 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   using PL = Plan<N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1, NB1 = E / R1;
+  using TW = TwLayout<M>;
+  constexpr int NTP = INV ? TW::NPRE : TW::NPOST;
   __shared__ cx<T> buf[M * TKP];
-  __shared__ cx<T> tq[N + 1];
+  __shared__ cx<T> ts[M];     // stage twiddles
+  __shared__ cx<T> tp[NTP];   // pre- (inverse) or post-processing (forward) twiddles
   T* rbuf = (T*)buf;  // real view: sample n of pencil p at ((n>>1)*TKP + p)*2 + (n&1)
   const Geo G = A.geo;
   const int tid = threadIdx.x;
@@ -117,49 +255,39 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   constexpr bool KV = uses_kv<LD, ST>();
   static_assert(!KV || AX == kAxX, "wavenumber operators run in x passes");
   constexpr bool XT_LDS =
-      KV && sizeof(cx<T>) * ((long)M * TKP + N + 1) + sizeof(double) * N <= 81920;
+      KV && sizeof(cx<T>) * ((long)M * TKP + M + NTP) + sizeof(double) * N <= 81920;
   __shared__ double xt[XT_LDS ? N : 1];
   const double* xsrc = nullptr;
   if constexpr (KV) xsrc = itab<LD, ST>(A.ld, A.waves);
   {
     const cx<T>* __restrict__ tg = (const cx<T>*)A.tw;
-    for (int u = tid; u <= N; u += BLOCK) tq[u] = tg[u];
+    for (int u = tid; u < M; u += BLOCK) ts[u] = tg[TW::STAGE + u];
+    for (int u = tid; u < NTP; u += BLOCK) tp[u] = tg[(INV ? TW::PRE : TW::POST) + u];
     if constexpr (XT_LDS) {
       for (int u = tid; u < N; u += BLOCK) xt[u] = xsrc[u];
       __syncthreads();
     }
   }
 
-  TileGeo<AX> tg{};
-  tg.nx = G.nx; tg.ny = G.ny; tg.nz = G.nz;
-  tg.nrows = G.nx * G.ny;
-  if constexpr (AX == kAxZ) {
-    tg.row0 = tile * TK;
-  } else {
-    const int tiles_k = (G.nz + TK - 1) / TK;
-    const int a = tile / tiles_k;
-    tg.k0 = (tile % tiles_k) * TK;
-    if constexpr (AX == kAxY) tg.t_i = a; else tg.t_j = a;
-  }
+  TileGeo<AX, N, TK, BLOCK> tg;
+  tg.init(G, tile, tid);
 
   // FFT-phase thread coordinates: pencil p, thread t within the pencil
   const int p = tid % TK, t = tid / TK;
   // wavenumbers of this thread's (j, k); an x-pass thread keeps its column
   // p = tid % TK in every phase
   Kv kv{};
-  if constexpr (KV) {
-    int i0, j0, k0;
-    long idx0;
-    tg.at(p, 0, i0, j0, k0, idx0);
-    kv = kv_thread(A.waves, j0, k0);
-  }
+  if constexpr (KV) kv = kv_thread(A.waves, tg.t_j, tg.kc);
   auto kv_at = [&](int i) {
     Kv K = kv;
     if constexpr (XT_LDS) K.xi = xt[i];
     else if constexpr (KV) K.xi = xsrc[i];
     return K;
   };
-  double dsum = 0.0;  // FP64 sum of the loaded values (mixed-precision DC modes)
+  // FP64 sum of the loaded values (mixed-precision DC modes): the forward
+  // z passes of the transforms whose (0,0,0) mode is kept in FP64
+  constexpr bool DSUM = !INV && AX == kAxZ && dc_capable(LD);
+  double dsum = 0.0;
   cx<T> v[E];
 
   if constexpr (!INV) {
@@ -170,37 +298,36 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
       // issued before any of its values is used
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+      WalkPos<AX, N, TK, TKP, BLOCK, 1> wp;
+      wp.init(tid);
 #pragma unroll 1
       for (int s0 = 0; s0 < PER; s0 += CH) {
         T xs[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          tg.at(q, n, i, j, k, idx);
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok;
+          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
           xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
-          if (A.part) dsum += (double)x;
-          if (dst && (n & 1)) x = -x;
-          const int vq = vpos<N>(n);
-          rbuf[((vq >> 1) * TKP + q) * 2 + (vq & 1)] = x;
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok, oddn;
+          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+          T x = ok ? xs[c] : (T)0;
+          if constexpr (DSUM) dsum += (double)x;
+          const int lp = wp.pos(tid, s0 + c, oddn);
+          if (dst & oddn) x = -x;
+          rbuf[lp] = x;
         }
       }
       __syncthreads();
       gather<T, M, E, TKP, R1>(v, buf, p, t);
     }
-    run_fft<T, N, M, E, TKP, -1, true>(typename PL::R{}, v, buf, p, t, tq);
+    run_fft<T, N, M, E, TKP, -1, true>(typename PL::R{}, v, buf, p, t, ts);
 
     // ---- DCT-II post-processing: Z -> Y, four outputs per pair item
     constexpr int NPRE = E / 2;
@@ -213,9 +340,9 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
       const cx<T> a = buf[kk * TKP + p], c = conj(buf[(kk ? M - kk : 0) * TKP + p]);
       const cx<T> Ev = a + c;
       const cx<T> Ov = mul_si<-1>(a - c);
-      const cx<T> wO = cmul(twq<T, N>(tq, 4 * kk), Ov);          // e^{-2 pi i k/N} O
-      const cx<T> Wk = cmul(twq<T, N>(tq, kk), Ev + wO);         // e^{-i pi k/2N} V_k
-      const cx<T> Wm = cmul(twq<T, N>(tq, M - kk), conj(Ev - wO));
+      const cx<T> wO = cmul(tp[3 * kk], Ov);                 // e^{-2 pi i k/N} O
+      const cx<T> Wk = cmul(tp[3 * kk + 1], Ev + wO);        // e^{-i pi k/2N} V_k
+      const cx<T> Wm = cmul(tp[3 * kk + 2], conj(Ev - wO));  // e^{-i pi (M-k)/2N} V_{M-k}
       y0[b] = Wk.x; y1[b] = -Wk.y; y2[b] = Wm.x; y3[b] = -Wm.y;
     }
     // z passes and store operators that read other fields write Y through
@@ -228,50 +355,74 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
         rbuf[((n >> 1) * TKP + p) * 2 + (n & 1)] = val;
       } else {
         int i, j, k;
-        long idx;
-        if (tg.at(p, n, i, j, k, idx)) store_eval<T, ST>(A.st, idx, val);
+        Ix idx;
+        tg.at(n, i, j, k, idx);
+        if (tg.colok) store_eval<T, ST>(A.st, idx, val);
+      }
+    };
+    // outputs o = kk, N-kk, M-kk, M+kk of item kk = t + C are t + K or K - t
+    // for constant K: positions from per-thread bases (the DST output n =
+    // N-1-o swaps the two families)
+    PencilPos<TKP> pp;
+    pp.init(p, t);
+    Ix gu = 0, gd = 0;
+    if constexpr (!STAGED) {
+      gu = tg.cbase + (Ix)t * tg.axs;
+      gd = tg.cbase - (Ix)t * tg.axs;
+    }
+    auto put_k = [&](bool up, int K, T val) {
+      if constexpr (STAGED) {
+        const int pa = up ? pp.up(K) : pp.dn(K);
+        const int pb = up ? pp.dn(N - 1 - K) : pp.up(N - 1 - K);
+        rbuf[dst ? pb : pa] = val;
+      } else {
+        const Ix ia = (up ? gu : gd) + (Ix)K * tg.axs;
+        const Ix ib = (up ? gd : gu) + (Ix)(N - 1 - K) * tg.axs;
+        if (tg.colok) store_eval<T, ST>(A.st, dst ? ib : ia, val);
       }
     };
     if constexpr (STAGED) __syncthreads();
 #pragma unroll
-    for (int b = 0; b <= NPRE; ++b) {
-      int kk = t + b * TP;
-      if (b == NPRE) kk = (t == 0) ? M / 2 : -1;
-      if (kk < 0) continue;
-      put(kk, y0[b]);
-      if (kk > 0) put(N - kk, y1[b]);
-      if (kk < M / 2) {
-        put(M - kk, y2[b]);
-        if (kk > 0) put(M + kk, y3[b]);
-      }
+    for (int b = 0; b < NPRE; ++b) {
+      const int C = b * TP;  // kk = t + C < M/2
+      put_k(true, C, y0[b]);
+      if (C > 0 || t > 0) put_k(false, N - C, y1[b]);
+      put_k(false, M - C, y2[b]);
+      if (C > 0 || t > 0) put_k(true, M + C, y3[b]);
+    }
+    if (t == 0) {  // kk = M/2
+      put(M / 2, y0[NPRE]);
+      put(N - M / 2, y1[NPRE]);
     }
     if constexpr (STAGED) {
       __syncthreads();
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = store_reads<ST>() ? (PER < 8 ? PER : 8) : PER;
+      DcIn dcv{};
+      if constexpr (store_reads<ST>()) dcv = dc_in(A.st);
+      WalkPos<AX, N, TK, TKP, BLOCK, 0> wp;
+      wp.init(tid);
 #pragma unroll 1
       for (int s0 = 0; s0 < PER; s0 += CH) {
         StIn in[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          tg.at(q, n, i, j, k, idx);
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok;
+          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
           in[c] = store_inputs<T, ST>(A.st, kv_at(i), idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          if (!tg.at(q, n, i, j, k, idx)) continue;
-          const T y = rbuf[((n >> 1) * TKP + q) * 2 + (n & 1)];
-          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i), idx, y, in[c]);
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok;
+          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+          if (!ok) continue;
+          bool oddn;
+          const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
+          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i), idx, y, in[c], dcv);
           else store_eval<T, ST>(A.st, idx, y);
         }
       }
@@ -281,30 +432,51 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
     {
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+      WalkPos<AX, N, TK, TKP, BLOCK, 0> wn;
+      WalkPos<AX, N, TK, TKP, BLOCK, 2> wr;
+      wn.init(tid);
+      wr.init(tid);
 #pragma unroll 1
       for (int s0 = 0; s0 < PER; s0 += CH) {
         T xs[CH];
+        if constexpr (load_split<LD>()) {
+          Raw4<T> raw[CH];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          tg.at(q, n, i, j, k, idx);
-          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
+          for (int c = 0; c < CH; ++c) {
+            int q, n, i, j, k;
+            Ix idx;
+            bool ok;
+            tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+            raw[c] = fetch<T, LD>(A.ld, G, i, j, k, idx);
+          }
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            int q, n, i, j, k;
+            Ix idx;
+            bool ok;
+            tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+            xs[c] = finish<T, LD>(raw[c], A.ld, A.phys, kv_at(i), G, i, j, k);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            int q, n, i, j, k;
+            Ix idx;
+            bool ok;
+            tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+            xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
+          }
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int e = tid + (s0 + c) * BLOCK;
-          int q, n;
-          if constexpr (AX == kAxZ) { q = e / N; n = e % N; } else { q = tid % TK; n = e / TK; }
-          int i, j, k;
-          long idx;
-          const T x = tg.at(q, n, i, j, k, idx) ? xs[c] : (T)0;
-          if (A.part) dsum += (double)x;
-          const int pos = dst ? (N - 1 - n) : n;
-          rbuf[((pos >> 1) * TKP + q) * 2 + (pos & 1)] = x;
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok;
+          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+          const T x = ok ? xs[c] : (T)0;
+          bool oddn;
+          const int pn = wn.pos(tid, s0 + c, oddn), pr = wr.pos(tid, s0 + c, oddn);
+          rbuf[dst ? pr : pn] = x;
         }
       }
     }
@@ -313,57 +485,66 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
     // Z_m = (V_m + V_{m+M}) + i e^{2 pi i m/N} (V_m - V_{m+M}),
     // V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m})
     const cx<T> r8 = {(T)0.70710678118654752440, (T)0.70710678118654752440};
-    auto X = [&](int n) -> T { return rbuf[((n >> 1) * TKP + p) * 2 + (n & 1)]; };
+    // samples m, N-m, M+m, M-m of item m = t + C from per-thread bases
+    PencilPos<TKP> pp;
+    pp.init(p, t);
 #pragma unroll
     for (int b = 0; b < NB1; ++b) {
-      const int jj = t + b * TP;
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
-        const int m = jj + r * (M / R1);
-        const T Xm = X(m), XNm = m ? X(N - m) : (T)0, XMp = X(M + m), XMm = X(M - m);
-        const cx<T> w1 = conj(twq<T, N>(tq, m));
+        const int C = b * TP + r * (M / R1);
+        const int m = t + C;
+        const T Xm = rbuf[pp.up(C)], XMp = rbuf[pp.up(M + C)], XMm = rbuf[pp.dn(M - C)];
+        T XNm;  // X_N = 0
+        if (C == 0) XNm = t == 0 ? (T)0 : rbuf[t == 0 ? pp.up(0) : pp.dn(N)];
+        else XNm = rbuf[pp.dn(N - C)];
+        const cx<T> w1 = tp[2 * m];  // e^{i pi m/2N}
         const cx<T> vk = cmul(w1, cx<T>{Xm, -XNm});
         const cx<T> vm = cmul(cmul(w1, r8), cx<T>{XMp, -XMm});
         const cx<T> Ev = vk + vm;
-        const cx<T> Ov = cmul(conj(twq<T, N>(tq, 4 * m)), vk - vm);
+        const cx<T> Ov = cmul(tp[2 * m + 1], vk - vm);  // e^{2 pi i m/N}
         v[b * R1 + r] = Ev + mul_si<1>(Ov);
       }
     }
     if constexpr (AX == kAxZ) {
-      run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, v, buf, p, t, tq);
+      run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, v, buf, p, t, ts);
       constexpr int PER = TK * N / BLOCK;
+      WalkPos<AX, N, TK, TKP, BLOCK, 1> wp;
+      wp.init(tid);
 #pragma unroll
       for (int s = 0; s < PER; ++s) {
-        const int e = tid + s * BLOCK;
-        const int q = e / N, n = e % N;
-        int i, j, k;
-        long idx;
-        if (!tg.at(q, n, i, j, k, idx)) continue;
-        const int vq = vpos<N>(n);
-        T y = rbuf[((vq >> 1) * TKP + q) * 2 + (vq & 1)];
-        if (dst && (n & 1)) y = -y;
+        int q, n, i, j, k;
+        Ix idx;
+        bool ok, oddn;
+        tg.walk(tid, s, q, n, i, j, k, idx, ok);
+        if (!ok) continue;
+        T y = rbuf[wp.pos(tid, s, oddn)];
+        if (dst & oddn) y = -y;
         store_eval<T, ST>(A.st, idx, y);
       }
     } else {
-      run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, v, buf, p, t, tq);
-      // last stage outputs stay in registers: value b*RL+r is slot m = t + b TP + r M/RL
+      run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, v, buf, p, t, ts);
+      // last stage outputs stay in registers: value b*RL+r is slot m = t + C,
+      // C = b TP + r M/RL, holding samples u = 2m + c at n = xpos(u):
+      // n = 4t + 2u0 (even) while u < M, else 2N-1-2u0-4t (odd), u0 = 2C + c
       constexpr int RL = Last<typename PL::R>::value, NBL = E / RL;
-      int i, j, k;
-      long idx;
-      if (tg.at(p, 0, i, j, k, idx)) {
+      if (tg.colok) {
+        const Ix g4u = tg.cbase + (Ix)(4 * t) * tg.axs, g4d = tg.cbase - (Ix)(4 * t) * tg.axs;
 #pragma unroll
         for (int b = 0; b < NBL; ++b) {
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
-            const int m = t + b * TP + r * (M / RL);
+            const int C = b * TP + r * (M / RL);
             const cx<T> z = v[b * RL + r];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-              const int n = xpos<N>(2 * m + c);
-              T y = c ? z.y : z.x;
-              if (dst && (n & 1)) y = -y;
-              tg.at(p, n, i, j, k, idx);
-              store_eval<T, ST>(A.st, idx, y);
+              const int u0 = 2 * C + c;
+              const T y = c ? z.y : z.x;
+              const Ix iu = g4u + (Ix)(2 * u0) * tg.axs;
+              const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
+              // constant when the thread range t < TP stays on one side of M
+              const bool low = (2 * (TP - 1) + u0 < M) ? true : (u0 >= M ? false : 2 * t + u0 < M);
+              store_eval<T, ST>(A.st, low ? iu : idn, (!low && dst) ? -y : y);
             }
           }
         }
@@ -372,7 +553,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   }
 
   // ---- FP64 block sum of the loaded values (deterministic order)
-  if (A.part) {
+  if constexpr (DSUM) {
+    if (!A.part) return;
     __syncthreads();
     double* red = (double*)buf;
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, 64);
