@@ -287,6 +287,8 @@ struct qsm_handle_s {
   int psi_cur = F_PSI_A, psi_old = F_PSI_B, rho_cur = F_RHO_A, rho_old = F_RHO_B;
   double* psihat = nullptr;   // FP64 state
   double* psihat_old = nullptr;
+  float* psihat32 = nullptr;  // mixed precision: psihat rounded to FP32 (the spectral prologues' source)
+  bool divv_valid = false;    // the D field holds IDCT(scale D-hat) of the current step
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
   double* dsc = nullptr;
@@ -325,10 +327,9 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
   if (!first_pass) rd = e;
   else switch (L.code) {
     case LD_PLAIN: rd = e; break;
-    case LD_PLAIN_D: case LD_COEF_F: case LD_COEF_G: case LD_SH: rd = 8; break;
+    case LD_COEF_F: case LD_COEF_G: case LD_SH: rd = e; break;
     case LD_ZERO: rd = 0; break;
-    case LD_GRAD: rd = e; break;
-    case LD_GRAD_D: case LD_LAPGRAD: rd = 8; break;
+    case LD_GRAD: case LD_LAPGRAD: rd = e; break;
     case LD_RHOSTAR: rd = 4 * e; break;
     case LD_DIVV: rd = 8 * e; break;
     case LD_STRESS: rd = 5 * e; break;
@@ -433,6 +434,12 @@ int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreO
 
 void* F(qsm_handle_s* h, int f) { return h->f[f]; }
 
+// psihat in the field precision: the FP64 state, or its FP32 copy in mixed
+// precision
+const void* psihat_src(const qsm_handle_s* h) {
+  return h->fp32 ? (const void*)h->psihat32 : (const void*)h->psihat;
+}
+
 int quasi_step(qsm_handle_s* h) {
   const long long n = ++h->step;
   const qsm_params& p = h->p;
@@ -440,21 +447,24 @@ int quasi_step(qsm_handle_s* h) {
   int rc;
   if (n >= 2) std::swap(h->rho_cur, h->rho_old);  // rho_old <- rho (quasi.cpp:1289)
   void *rho = F(h, h->rho_cur), *rho_old = F(h, h->rho_old), *psi = F(h, h->psi_cur);
+  const void* psih = psihat_src(h);
+  const bool dlive = n > p.divv_switch;  // D is computed from this step on
+  h->divv_valid = false;
 
   // S1: F, G (quasi.cpp:1034-1057)
   LoadOp L{};
-  L.code = LD_COEF_F; L.a[0] = h->psihat;
+  L.code = LD_COEF_F; L.a[0] = psih;
   if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
   L.code = LD_COEF_G;
   if ((rc = inv3(h, -1, L, F(h, F_G)))) return rc;
 
   // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD_D; L.axis = c; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
     L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = F(h, F_RHOHAT);
     if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
   }
 
@@ -466,11 +476,16 @@ int quasi_step(qsm_handle_s* h) {
   if ((rc = fwd3(h, -1, L, F(h, F_RHOHAT), S, mixed ? D_RHOSUM : -1))) return rc;
   if ((rc = inv3(h, -1, plain_load(F(h, F_RHOHAT)), rho))) return rc;
 
-  // S4: divergence (quasi.cpp:1273-1293)
-  L = LoadOp{}; L.code = n > p.divv_switch ? LD_DIVV : LD_ZERO;
-  L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
-  L.a[5] = F(h, F_RX); L.a[6] = F(h, F_RY); L.a[7] = F(h, F_RZ);
-  if ((rc = fwd3(h, -1, L, F(h, F_DHAT), plain_store(F(h, F_DHAT)), -1))) return rc;
+  // S4: divergence (quasi.cpp:1273-1293).  Up to divv_switch the reference
+  // transforms an all-zero D; D-hat = 0 then, which the D-hat array already
+  // holds (it is zeroed at set-up and first written at step divv_switch+1),
+  // so the transform of zeros is not run.
+  if (dlive) {
+    L = LoadOp{}; L.code = LD_DIVV;
+    L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
+    L.a[5] = F(h, F_RX); L.a[6] = F(h, F_RY); L.a[7] = F(h, F_RZ);
+    if ((rc = fwd3(h, -1, L, F(h, F_DHAT), plain_store(F(h, F_DHAT)), -1))) return rc;
+  }
 
   // S5: stress (quasi.cpp:1298-1330)
   for (int c = 0; c < 3; ++c) {
@@ -488,13 +503,14 @@ int quasi_step(qsm_handle_s* h) {
   const int n0 = p.compat_ranks > 0 ? p.nx / p.compat_ranks : 0;
   for (int c = 0; c < 3; ++c) {
     L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
-    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = F(h, F_DHAT);
+    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
+    L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
     if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
   }
 
-  // S8: divv output (quasi.cpp:1888-1900)
-  L = LoadOp{}; L.code = LD_SCALE; L.a[0] = F(h, F_DHAT);
-  if ((rc = inv3(h, -1, L, F(h, F_DIVV)))) return rc;
+  // S8: the D output, IDCT(scale D-hat) (quasi.cpp:1888-1900), is read by no
+  // later stage of the step; it is formed when the field is asked for
+  // (divv_output).
 
   // S9 + S10: N, its transform, and the update fused into the last pass
   L = LoadOp{}; L.code = n > p.divv_switch + 1 ? LD_NONLIN_P : LD_NONLIN;
@@ -503,16 +519,26 @@ int quasi_step(qsm_handle_s* h) {
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
   L.a[10] = F(h, F_P); L.a[11] = F(h, F_RX); L.a[12] = F(h, F_RY); L.a[13] = F(h, F_RZ);
   S = StoreOp{}; S.code = ST_UPDATE; S.first = n == 1;
-  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.s[0] = h->psihat;
+  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.d[2] = h->psihat32; S.s[0] = h->psihat;
   if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
   std::swap(h->psihat, h->psihat_old);
 
   // psi = IDCT(psihat) into the psi_old buffer, then swap (psi_old <- psi)
-  L = LoadOp{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
-  if ((rc = inv3(h, -1, L, F(h, h->psi_old)))) return rc;
+  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_old)))) return rc;
   std::swap(h->psi_cur, h->psi_old);
   return QSM_OK;
+}
+
+// The D field: IDCT(scale D-hat) of the current step (quasi.cpp:1888-1900),
+// formed on request.  Zero while D-hat is (steps <= divv_switch).
+int divv_output(qsm_handle_s* h) {
+  if (h->divv_valid) return QSM_OK;
+  LoadOp L{};
+  L.code = LD_SCALE; L.a[0] = F(h, F_DHAT);
+  const int rc = inv3(h, -1, L, F(h, F_DIVV));
+  if (!rc) h->divv_valid = true;
+  return rc;
 }
 
 int stb_step(qsm_handle_s* h) {
@@ -522,10 +548,11 @@ int stb_step(qsm_handle_s* h) {
   void* psi = F(h, h->psi_cur);
   // sh = IDCT(-aLin psihat) (quasi-stb.cpp:1015-1038), kept in F
   LoadOp L{};
-  L.code = LD_SH; L.a[0] = h->psihat;
+  const void* psih = psihat_src(h);
+  L.code = LD_SH; L.a[0] = psih;
   if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD_D; L.axis = c; L.a[0] = h->psihat;
+    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
   }
   for (int c = 0; c < 3; ++c) {
@@ -543,12 +570,11 @@ int stb_step(qsm_handle_s* h) {
   L.a[0] = psi; L.a[1] = F(h, F_VX); L.a[2] = F(h, F_VY); L.a[3] = F(h, F_VZ);
   L.a[4] = F(h, F_GX); L.a[5] = F(h, F_GY); L.a[6] = F(h, F_GZ);
   StoreOp S{}; S.code = ST_UPDATE_STB; S.first = n == 1;
-  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.s[0] = h->psihat;
+  S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.d[2] = h->psihat32; S.s[0] = h->psihat;
   if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
   std::swap(h->psihat, h->psihat_old);
-  L = LoadOp{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
-  if ((rc = inv3(h, -1, L, F(h, h->psi_old)))) return rc;
+  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_old)))) return rc;
   std::swap(h->psi_cur, h->psi_old);
   return QSM_OK;
 }
@@ -579,6 +605,7 @@ void free_handle(qsm_handle_s* h) {
   for (int f = 0; f < F_COUNT; ++f) if (h->f[f]) hipFree(h->f[f]);
   if (h->psihat) hipFree(h->psihat);
   if (h->psihat_old) hipFree(h->psihat_old);
+  if (h->psihat32) hipFree(h->psihat32);
   if (h->part) hipFree(h->part);
   if (h->dsc) hipFree(h->dsc);
   for (auto& g : h->graphs) hipGraphExecDestroy(g.second);
@@ -694,6 +721,7 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     if (hipMalloc(&h->f[f], h->n * h->esz) != hipSuccess) return bail(QSM_ERR_NOMEM, "field");
   if (hipMalloc(&h->psihat, h->n * 8) != hipSuccess || hipMalloc(&h->psihat_old, h->n * 8) != hipSuccess)
     return bail(QSM_ERR_NOMEM, "psihat");
+  if (h->fp32 && hipMalloc(&h->psihat32, h->n * 4) != hipSuccess) return bail(QSM_ERR_NOMEM, "psihat32");
   long nb = pass_blocks(h->fp32, p.nz, 2, h->geo);
   h->part_n = nb > 4096 ? nb : 4096;
   if (hipMalloc(&h->part, h->part_n * 4 * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "partials");
@@ -729,9 +757,9 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   h->fp32 = saved_fp32;
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(h->psihat_old, h->psihat, n * 8, hipMemcpyDeviceToDevice, h->stream));
+  if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)n, h->stream));
   // psi = IDCT(psihat) (quasi.cpp:924-927), psi_old = psi
-  LoadOp L{}; L.code = LD_PLAIN_D; L.a[0] = h->psihat;
-  if ((rc = inv3(h, -1, L, F(h, h->psi_cur)))) return rc;
+  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_cur)))) return rc;
   HIPCHK(hipMemcpyAsync(F(h, h->psi_old), F(h, h->psi_cur), n * h->esz, hipMemcpyDeviceToDevice, h->stream));
   if (h->p.variant == QSM_STB) {
     // rho = 1 in the pure smectic (quasi-stb.cpp:1232): kept for the mass diagnostic
@@ -746,6 +774,7 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   HIPCHK(hipStreamSynchronize(h->stream));
   h->step = 0;
   h->growth = 0;
+  h->divv_valid = false;
   h->ready = true;
   return QSM_OK;
 }
@@ -879,7 +908,12 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
     case QSM_VY: src = F(h, F_VY); break;
     case QSM_VZ: src = F(h, F_VZ); break;
     case QSM_P: src = F(h, F_P); break;
-    case QSM_DIVV: src = F(h, F_DIVV); break;
+    case QSM_DIVV: {
+      const int rc = divv_output(h);
+      if (rc) return rc;
+      src = F(h, F_DIVV);
+      break;
+    }
     case QSM_PSIHAT: src = h->psihat; is_double = true; break;
     case QSM_PSIHAT_OLD: src = h->psihat_old; is_double = true; break;
     case QSM_F: src = F(h, F_F); break;
@@ -1280,7 +1314,9 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
     if (f != F_PSI_A && f != F_PSI_B && f != F_RHO_A && f != F_RHO_B) get(h->f[f], h->n * h->esz);
   fclose(fp);
   if (!ok) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is truncated", path);
+  if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)h->n, h->stream));
   h->step = hd.step;
+  h->divv_valid = false;
   h->ready = true;
   if (h->p.variant == QSM_STB) return stb_growth(h);
   return QSM_OK;

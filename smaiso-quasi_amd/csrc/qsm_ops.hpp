@@ -33,14 +33,15 @@ struct Phys {
   int pad_;
 };
 
+// Every source is an array of the field type T.  The spectral prologues
+// read psihat in the field precision: the FP64 state itself, or in mixed
+// precision its FP32 copy (rounded once by the update that writes it).
 enum LoadCode : int {
-  LD_PLAIN = 0,     // a0[idx] (field type T)
-  LD_PLAIN_D,       // (T) a0[idx], a0 is double (psihat)
-  LD_ZERO,          // 0 (divergence before divv_switch: the reference transforms zeros)
+  LD_PLAIN = 0,     // a0[idx]
+  LD_ZERO,          // 0 (the reference transforms zeros: D before divv_switch)
   LD_COEF_F,        // alpha (q0^2 - K^2) psihat                          quasi.cpp:1042
   LD_COEF_G,        // -K^2 alpha (q0^2 - K^2) psihat                     quasi.cpp:1044-1045
-  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1 (src: T) quasi.cpp:1124-1171
-  LD_GRAD_D,        // same with a double source (psihat)
+  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1           quasi.cpp:1124-1171
   LD_LAPGRAD,       // -K_c^2 (-s_c[m] psihat[m+1])                       quasi.cpp:1176-1191
   LD_SH,            // -aLin psihat (stb)                                 quasi-stb.cpp:1023
   LD_RHOSTAR,       // kp sqrt(q0^2 psi^2 + |g|^2) + rho0                 quasi.cpp:1243-1247
@@ -67,7 +68,7 @@ enum StoreCode : int {
 
 struct LoadOp {
   int code;
-  int axis;      // component c for GRAD/GRAD_D/LAPGRAD
+  int axis;      // component c for GRAD/LAPGRAD
   int flag;      // unused (reserved)
   int n0;        // VEL0-2: slab size of the P-rank halo compat map (0 = exact)
   const void* a[14];
@@ -132,7 +133,7 @@ __device__ __forceinline__ const double* itab(const LoadOp& L, const Waves& W) {
   if constexpr (LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
                 LD == LD_VEL1 || LD == LD_VEL2) {
     return W.qx;
-  } else if constexpr (LD == LD_VEL0 || LD == LD_GRAD || LD == LD_GRAD_D) {
+  } else if constexpr (LD == LD_VEL0 || LD == LD_GRAD) {
     return W.sx;
   } else if constexpr (LD == LD_LAPGRAD) {
     return L.axis == 0 ? W.sx : W.qx;
@@ -149,7 +150,7 @@ template <int LD, int ST>
 constexpr bool uses_kv() {
   return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
          LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_GRAD ||
-         LD == LD_GRAD_D || LD == LD_LAPGRAD || ST == ST_UPDATE || ST == ST_UPDATE_STB ||
+         LD == LD_LAPGRAD || ST == ST_UPDATE || ST == ST_UPDATE_STB ||
          ST == ST_FILTER;
 }
 
@@ -287,29 +288,25 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
                                        const Geo& G, int i, int j, int k, Ix idx) {
   if constexpr (LD == LD_PLAIN) {
     return ld<T>(L.a[0], idx);
-  } else if constexpr (LD == LD_PLAIN_D) {
-    return (T)ld<double>(L.a[0], idx);
   } else if constexpr (LD == LD_ZERO) {
     return (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
     const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
-    const double fh = P.alpha * (P.q02 - mq2) * ld<double>(L.a[0], idx);
+    const double fh = P.alpha * (P.q02 - mq2) * (double)ld<T>(L.a[0], idx);
     return (T)(LD == LD_COEF_F ? fh : -mq2 * fh);
   } else if constexpr (LD == LD_GRAD) {
     return (T)grad_hat<T>(L, G, K, i, j, k, idx);
-  } else if constexpr (LD == LD_GRAD_D) {
-    return (T)grad_hat<double>(L, G, K, i, j, k, idx);
   } else if constexpr (LD == LD_LAPGRAD) {
     // i-table: s_x when c = 0, else q_x
     const int c = L.axis;
     const double ay = c == 1 ? K.sy : K.qy;
     const double az = c == 2 ? K.sz : K.qz;
     const double m = sq(K.xi) + sq(ay) + sq(az);
-    return (T)(-m * grad_hat<double>(L, G, K, i, j, k, idx));
+    return (T)(-m * grad_hat<T>(L, G, K, i, j, k, idx));
   } else if constexpr (LD == LD_SH) {
     const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
     const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
-    return (T)(-alin * ld<double>(L.a[0], idx));
+    return (T)(-alin * (double)ld<T>(L.a[0], idx));
   } else if constexpr (LD == LD_RHOSTAR) {
     const T ps = ld<T>(L.a[0], idx), gx = ld<T>(L.a[1], idx),
             gy = ld<T>(L.a[2], idx), gz = ld<T>(L.a[3], idx);
@@ -408,7 +405,9 @@ __device__ __forceinline__ DcIn dc_in(const StoreOp& S) {
 }
 
 // d0: psihat_old (double, in) -> psihat_new (out); d1: N_old (T, in) -> N (out);
-// s0: psihat (double, in).  quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
+// s0: psihat (double, in); mixed precision (T = float): d2 receives psihat_new
+// rounded to FP32, the copy the next step's spectral prologues read.
+// quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
 template <typename T, int ST>
 __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Kv& K, Ix idx,
                                             T v, const StIn& in, const DcIn& D) {
@@ -435,6 +434,7 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
   }
   st<double>(S.d[0], idx, nw);
   st<T>(S.d[1], idx, (T)nh);
+  if constexpr (sizeof(T) < sizeof(double)) st<T>(S.d[2], idx, (T)nw);
   if (z && S.dc_out) S.dc_out[0] = nh;
 }
 

@@ -182,4 +182,19 @@ hipError_t launch_finish(const double* part, long nparts, double* out, hipStream
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) k_round32(const double* __restrict__ src,
+                                                 float* __restrict__ dst, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (float)src[i];
+}
+
+hipError_t launch_round32(const double* src, float* dst, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long nb = (n + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(k_round32, dim3((unsigned)nb), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
 }  // namespace qsm

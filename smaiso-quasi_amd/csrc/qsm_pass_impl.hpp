@@ -568,9 +568,21 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: workgroup b runs on XCD b mod 8.  Tile order is remapped so that XCD x
+// gets one contiguous range of tiles; neighbouring tiles, which read each
+// other's edge cache lines in the shifted-neighbour operators and share
+// 128-byte lines in the half-width variant, then meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int NX = 8;
+  const int x = b % NX, s = b / NX;
+  const int per = nb / NX, rem = nb % NX;
+  return x * per + (x < rem ? x : rem) + s;
+}
+
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const PassArgs A) {
-  pass_tile<T, N, AX, INV, LD, ST, V>(A, (int)blockIdx.x);
+  pass_tile<T, N, AX, INV, LD, ST, V>(A, xcd_tile((int)blockIdx.x, (int)gridDim.x));
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
@@ -579,7 +591,7 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
     k_pass_batch(const PassArgs* __restrict__ descs, const int* __restrict__ prefix, int count) {
-  const int b = (int)blockIdx.x;
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   int lo = 0, hi = count;  // invariant: prefix[lo] <= b < prefix[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -652,8 +664,7 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
     if (inv && st == ST_PLAIN) {
       switch (ld) {
 #define QSM_L(OP) case OP: return launch_one<T, N, AX, true, OP, ST_PLAIN>(A, nb, s, B);
-        QSM_L(LD_PLAIN_D) QSM_L(LD_COEF_F) QSM_L(LD_COEF_G) QSM_L(LD_GRAD) QSM_L(LD_GRAD_D)
-        QSM_L(LD_LAPGRAD) QSM_L(LD_SH) QSM_L(LD_PRESSURE) QSM_L(LD_VEL0) QSM_L(LD_VEL1)
+        QSM_L(LD_COEF_F) QSM_L(LD_COEF_G) QSM_L(LD_GRAD) QSM_L(LD_LAPGRAD) QSM_L(LD_SH) QSM_L(LD_PRESSURE) QSM_L(LD_VEL0) QSM_L(LD_VEL1)
         QSM_L(LD_VEL2) QSM_L(LD_SCALE)
 #undef QSM_L
         default: break;
