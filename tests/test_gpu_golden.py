@@ -4,10 +4,13 @@ written by tests/golden/make_interface256_golden.py), compared on the
 fixture's uniform 1-in-8 subsample of psi, rho and v.
 
 Tolerances: 1e-10 relative L2 for psi and rho at every snapshot, and for v
-while the divergence term is off (steps <= divv_switch).  Once it is on, v is
-the small solenoidal remainder of the stress and amplifies round-off by about
-1e6 (measured in the oracle itself: a 1e-16 perturbation of psi moves v by
-~1e-10 within two steps), so v is held to 1e-7 there.
+while the divergence term is off (steps <= divv_switch).  Once it is on, v
+carries the divergence D = -((rho - rho_old)/dt + v.grad rho)/rho_mean, a
+cancellation of nearly equal densities fed back through v, and round-off in
+v grows without touching psi: in the oracle itself a 1e-16 relative
+perturbation of psi moves v_x by 1e-8 .. 1e-7 within 100 steps after the
+switch while psi moves by 4e-15 (64^3 interface workload, divv_switch 20).
+v is held to 1e-6 there, ten times that sensitivity.
 """
 import glob
 import os
@@ -41,7 +44,7 @@ def test_interface256_fp64_matches_oracle(qsm_mod):
         sim.step(n - done)
         done = n
         st = int(fx["stride"])
-        v_tol = 1e-10 if n <= d["divv_switch"] else 1e-7
+        v_tol = 1e-10 if n <= d["divv_switch"] else 1e-6
         for f in ("PSI", "RHO", "VX", "VY", "VZ"):
             got = sim.field(f)
             e = rel_l2(got[::st, ::st, ::st], fx[f])

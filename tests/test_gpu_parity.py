@@ -270,7 +270,7 @@ def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec):
 
 @pytest.mark.parametrize("prec", (0, 1))
 def test_tile_variants_agree(qsm_mod, prec):
-    """The half-width tile variant of every plain pass (chosen by the
+    """The half-width tile variant of the plain passes (chosen by the
     autotuner) regroups pencils into workgroups without changing the
     operations on any pencil.  It is a separate instantiation, so the
     compiler may contract a different set of FP64 multiply-adds: 1 ulp, which
