@@ -569,10 +569,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2: workgroup b runs on XCD b mod 8.  Tile order is remapped so that XCD x
-// gets one contiguous range of tiles; neighbouring tiles, which read each
-// other's edge cache lines in the shifted-neighbour operators and share
-// 128-byte lines in the half-width variant, then meet in the same L2.
+// L2: workgroup b runs on XCD b mod 8.  xcd_tile gives XCD x one contiguous
+// range of tiles (any tile count).
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
   constexpr int NX = 8;
   const int x = b % NX, s = b / NX;
@@ -580,9 +578,33 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + (x < rem ? x : rem) + s;
 }
 
+// Tile order of a single pass, measured at 512^3:
+//  * strided passes: the G column blocks of one row (x pass) or slab (y pass)
+//    form a unit that runs on one XCD, units going round-robin over the XCDs,
+//    so neighbouring columns (shifted-neighbour reads) meet in one L2 while
+//    the chip as a whole sweeps rows/slabs in increasing order;
+//  * z passes: rows in decreasing order.  The y pass before a z pass (inverse
+//    transforms) ends on the high slabs, and the y pass after one (forward
+//    transforms) starts on the low slabs, so the z pass begins, and the next
+//    y pass begins, on the data written last, still in the 256 MB
+//    memory-side cache.
+template <typename T, int N, int AX, int V>
+__device__ __forceinline__ int tile_order(const Geo& g, int b, int nb) {
+  if constexpr (AX == kAxZ) {
+    return nb - 1 - b;
+  } else {
+    constexpr int NX = 8;
+    const int G = (g.nz + TileCfg<T, N, AX, V>::TK - 1) / TileCfg<T, N, AX, V>::TK;
+    if (nb % (NX * G) != 0) return xcd_tile(b, nb);
+    const int x = b % NX, s = b / NX;
+    return ((s / G) * NX + x) * G + s % G;
+  }
+}
+
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const PassArgs A) {
-  pass_tile<T, N, AX, INV, LD, ST, V>(A, xcd_tile((int)blockIdx.x, (int)gridDim.x));
+  pass_tile<T, N, AX, INV, LD, ST, V>(
+      A, tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x));
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
