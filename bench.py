@@ -39,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # every pointwise stage fused into a transform pass): 194 field sweeps
 SWEEPS_PER_STEP = 194
 
-LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad", "lapgrad", "sh", "rhostar", "divv",
+LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "lapgrad_x",
+            "lapgrad_y", "lapgrad_z", "sh", "rhostar", "divv",
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
             "stress_stb", "nonlin_stb", "nonlin_stb0"]
 STORE_OPS = ["plain", "filter", "update", "update_stb"]

@@ -329,7 +329,8 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
     case LD_PLAIN: rd = e; break;
     case LD_COEF_F: case LD_COEF_G: case LD_SH: rd = e; break;
     case LD_ZERO: rd = 0; break;
-    case LD_GRAD: case LD_LAPGRAD: rd = e; break;
+    case LD_GRAD_X: case LD_GRAD_Y: case LD_GRAD_Z:
+    case LD_LAPGRAD_X: case LD_LAPGRAD_Y: case LD_LAPGRAD_Z: rd = e; break;
     case LD_RHOSTAR: rd = 4 * e; break;
     case LD_DIVV: rd = 8 * e; break;
     case LD_STRESS: rd = 5 * e; break;
@@ -460,11 +461,11 @@ int quasi_step(qsm_handle_s* h) {
 
   // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = psih;
+    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = F(h, F_RHOHAT);
+    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
     if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_LAPGRAD; L.axis = c; L.a[0] = psih;
+    L = LoadOp{}; L.code = LD_LAPGRAD_X + c; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
   }
 
@@ -552,7 +553,7 @@ int stb_step(qsm_handle_s* h) {
   L.code = LD_SH; L.a[0] = psih;
   if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
   for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD; L.axis = c; L.a[0] = psih;
+    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
     if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
   }
   for (int c = 0; c < 3; ++c) {

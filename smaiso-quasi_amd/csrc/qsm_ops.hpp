@@ -41,8 +41,12 @@ enum LoadCode : int {
   LD_ZERO,          // 0 (the reference transforms zeros: D before divv_switch)
   LD_COEF_F,        // alpha (q0^2 - K^2) psihat                          quasi.cpp:1042
   LD_COEF_G,        // -K^2 alpha (q0^2 - K^2) psihat                     quasi.cpp:1044-1045
-  LD_GRAD,          // -s_c[m] src[m+1 along c], 0 at m = N_c-1           quasi.cpp:1124-1171
-  LD_LAPGRAD,       // -K_c^2 (-s_c[m] psihat[m+1])                       quasi.cpp:1176-1191
+  LD_GRAD_X,        // -s_c[m] src[m+1 along c], 0 at m = N_c-1, c = x   quasi.cpp:1124-1171
+  LD_GRAD_Y,        //   c = y
+  LD_GRAD_Z,        //   c = z
+  LD_LAPGRAD_X,     // -K_c^2 (-s_c[m] psihat[m+1]), c = x               quasi.cpp:1176-1191
+  LD_LAPGRAD_Y,     //   c = y
+  LD_LAPGRAD_Z,     //   c = z
   LD_SH,            // -aLin psihat (stb)                                 quasi-stb.cpp:1023
   LD_RHOSTAR,       // kp sqrt(q0^2 psi^2 + |g|^2) + rho0                 quasi.cpp:1243-1247
   LD_DIVV,          // -((rho-rho_old)/dt + v.r)/(0.5(rho+rho_old))       quasi.cpp:1280-1285
@@ -68,7 +72,7 @@ enum StoreCode : int {
 
 struct LoadOp {
   int code;
-  int axis;      // component c for GRAD/LAPGRAD
+  int axis;      // component c (informative: STRESS)
   int flag;      // unused (reserved)
   int n0;        // VEL0-2: slab size of the P-rank halo compat map (0 = exact)
   const void* a[14];
@@ -117,61 +121,56 @@ __device__ __forceinline__ double rd_if(const void* p, Ix idx, int off, bool ok)
   return ok ? v : 0.0;
 }
 
-// Wavenumber values at one element of an x pass.  An x-pass thread keeps one
-// (j, k) for all its elements, so the y and z entries are read once per
-// thread; xi is the launch's i-table at the element's i (itab() below).
+// Wavenumbers at one element: q_c = pi m_c/L_c, s_c = pi (m_c+1)/L_c and the
+// density filter factor f_c = exp(-filter q_c^2) per axis (quasi.cpp:397-431,
+// 1261).  A pass fills the entries of its own axis per element (from a table
+// it stages) and the others once per thread or per row.
 struct Kv {
-  double xi;
-  double qy, sy, qz, sz, fy, fz;
+  double qx, sx, fx, qy, sy, fy, qz, sz, fz;
 };
-
-// The one i-indexed table a launch's operator reads (uniform per launch),
-// or null: q_x or s_x for the spectral prologues, q_x for the update and the
-// filter factors f_x for the density filter.
-template <int LD, int ST>
-__device__ __forceinline__ const double* itab(const LoadOp& L, const Waves& W) {
-  if constexpr (LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
-                LD == LD_VEL1 || LD == LD_VEL2) {
-    return W.qx;
-  } else if constexpr (LD == LD_VEL0 || LD == LD_GRAD) {
-    return W.sx;
-  } else if constexpr (LD == LD_LAPGRAD) {
-    return L.axis == 0 ? W.sx : W.qx;
-  } else if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
-    return W.qx;
-  } else if constexpr (ST == ST_FILTER) {
-    return W.fx;
-  } else {
-    return nullptr;
-  }
-}
 
 template <int LD, int ST>
 constexpr bool uses_kv() {
   return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
-         LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || LD == LD_GRAD ||
-         LD == LD_LAPGRAD || ST == ST_UPDATE || ST == ST_UPDATE_STB ||
-         ST == ST_FILTER;
+         LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || (LD >= LD_GRAD_X && LD <= LD_LAPGRAD_Z) ||
+         ST == ST_UPDATE || ST == ST_UPDATE_STB || ST == ST_FILTER;
 }
 
-__device__ __forceinline__ Kv kv_thread(const Waves& W, int j, int k) {
-  return Kv{0.0, W.qy[j], W.sy[j], W.qz[k], W.sz[k], W.fy[j], W.fz[k]};
+// Which wavenumbers of axis c an operator reads: q_c, s_c, f_c.
+template <int LD, int ST>
+constexpr bool uses_q(int c) {
+  return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
+         ST == ST_UPDATE || ST == ST_UPDATE_STB ||
+         (LD == LD_VEL0 && c != 0) || (LD == LD_VEL1 && c != 1) || (LD == LD_VEL2 && c != 2) ||
+         (LD >= LD_LAPGRAD_X && LD <= LD_LAPGRAD_Z && c != LD - LD_LAPGRAD_X);
 }
+template <int LD, int ST>
+constexpr bool uses_s(int c) {
+  return (LD == LD_VEL0 && c == 0) || (LD == LD_VEL1 && c == 1) || (LD == LD_VEL2 && c == 2) ||
+         (LD >= LD_GRAD_X && LD <= LD_GRAD_Z && c == LD - LD_GRAD_X) ||
+         (LD >= LD_LAPGRAD_X && LD <= LD_LAPGRAD_Z && c == LD - LD_LAPGRAD_X);
+}
+template <int LD, int ST>
+constexpr bool uses_f(int) {
+  return ST == ST_FILTER;
+}
+// pass-axis tables staged for an operator (bit 0: q, 1: s, 2: f)
+template <int AX, int LD, int ST>
+constexpr int kv_tables() {
+  return (uses_q<LD, ST>(AX) ? 1 : 0) | (uses_s<LD, ST>(AX) ? 2 : 0) | (uses_f<LD, ST>(AX) ? 4 : 0);
+}
+constexpr int kv_count(int mask) { return (mask & 1) + ((mask >> 1) & 1) + ((mask >> 2) & 1); }
 
-// Shifted read along axis c used by the gradient stencils: -s_c[m] src[m+1],
-// 0 at m = N_c - 1.  The axis is uniform for a launch, so the per-element
-// choices below are selects, not branches.
-template <typename S>
+// Shifted read along axis C used by the gradient stencils: -s_C[m] src[m+1],
+// 0 at m = N_C - 1.
+template <typename S, int C>
 __device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const Kv& K, int i,
                                            int j, int k, Ix idx) {
-  const int nyz = G.ny * G.nz;
-  const int c = L.axis;
-  const int m = c == 0 ? i : (c == 1 ? j : k);
-  const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
-  const int o = c == 0 ? nyz : (c == 1 ? G.nz : 1);
-  const double st = c == 0 ? K.xi : (c == 1 ? K.sy : K.sz);
-  const bool ok = m < nc - 1;
-  return -st * rd_if<S>(L.a[0], idx, o, ok);
+  const int m = C == 0 ? i : (C == 1 ? j : k);
+  const int nc = C == 0 ? G.nx : (C == 1 ? G.ny : G.nz);
+  const int o = C == 0 ? G.ny * G.nz : (C == 1 ? G.nz : 1);
+  const double st = C == 0 ? K.sx : (C == 1 ? K.sy : K.sz);
+  return -st * rd_if<S>(L.a[0], idx, o, m < nc - 1);
 }
 
 // The projection prologues (pressure and the three velocity components) read
@@ -254,29 +253,29 @@ __device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phy
                c = h.fc ? (double)r.c : 0.0, dd = h.fd ? (double)r.d : 0.0;
   const double nu = P.nu;
   if constexpr (LD == LD_PRESSURE) {
-    // quasi.cpp:1385-1415; i-table q_x
-    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
-    const double pv = P.scale * (-(K.xi * a + K.qy * b + K.qz * c) / mq2);
+    // quasi.cpp:1385-1415
+    const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
+    const double pv = P.scale * (-(K.qx * a + K.qy * b + K.qz * c) / mq2);
     return (T)((i + j + k == 0) ? 0.0 : pv);
   } else if constexpr (LD == LD_VEL0) {
-    // global-index form of quasi.cpp:1522-1634; i-table s_x
+    // global-index form of quasi.cpp:1522-1634
     const bool in = i < G.nx - 1;
-    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.qz));
-    const double cm1 = (P.scale / nu) * rm2, cm2 = K.xi * rm2;
-    const double dot = K.xi * a + K.qy * b + K.qz * c;
-    const double vx = cm1 * (a - cm2 * dot + nu * K.xi * dd);
+    const double rm2 = 1.0 / (sq(K.sx) + sq(K.qy) + sq(K.qz));
+    const double cm1 = (P.scale / nu) * rm2, cm2 = K.sx * rm2;
+    const double dot = K.sx * a + K.qy * b + K.qz * c;
+    const double vx = cm1 * (a - cm2 * dot + nu * K.sx * dd);
     return (T)(in ? vx : 0.0);
   } else if constexpr (LD == LD_VEL1) {
-    // quasi.cpp:1681-1872; i-table q_x
+    // quasi.cpp:1681-1872
     const bool in = (j < G.ny - 1) & (k < G.nz - 1);
-    const double rm2 = 1.0 / (sq(K.xi) + sq(K.sy) + sq(K.qz));
-    const double dot = K.xi * b + K.sy * a + K.qz * c;
+    const double rm2 = 1.0 / (sq(K.qx) + sq(K.sy) + sq(K.qz));
+    const double dot = K.qx * b + K.sy * a + K.qz * c;
     const double vv = ((P.scale / nu) * rm2) * (a - (K.sy * rm2) * dot + nu * K.sy * dd);
     return (T)(in ? vv : 0.0);
   } else {
     const bool in = (j < G.ny - 1) & (k < G.nz - 1);
-    const double rm2 = 1.0 / (sq(K.xi) + sq(K.qy) + sq(K.sz));
-    const double dot = K.xi * b + K.qy * c + K.sz * a;
+    const double rm2 = 1.0 / (sq(K.qx) + sq(K.qy) + sq(K.sz));
+    const double dot = K.qx * b + K.qy * c + K.sz * a;
     const double vv = ((P.scale / nu) * rm2) * (a - (K.sz * rm2) * dot + nu * K.sz * dd);
     return (T)(in ? vv : 0.0);
   }
@@ -291,20 +290,20 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
   } else if constexpr (LD == LD_ZERO) {
     return (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
-    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
+    const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
     const double fh = P.alpha * (P.q02 - mq2) * (double)ld<T>(L.a[0], idx);
     return (T)(LD == LD_COEF_F ? fh : -mq2 * fh);
-  } else if constexpr (LD == LD_GRAD) {
-    return (T)grad_hat<T>(L, G, K, i, j, k, idx);
-  } else if constexpr (LD == LD_LAPGRAD) {
-    // i-table: s_x when c = 0, else q_x
-    const int c = L.axis;
+  } else if constexpr (LD >= LD_GRAD_X && LD <= LD_GRAD_Z) {
+    return (T)grad_hat<T, LD - LD_GRAD_X>(L, G, K, i, j, k, idx);
+  } else if constexpr (LD >= LD_LAPGRAD_X && LD <= LD_LAPGRAD_Z) {
+    constexpr int c = LD - LD_LAPGRAD_X;
+    const double ax = c == 0 ? K.sx : K.qx;
     const double ay = c == 1 ? K.sy : K.qy;
     const double az = c == 2 ? K.sz : K.qz;
-    const double m = sq(K.xi) + sq(ay) + sq(az);
-    return (T)(-m * grad_hat<T>(L, G, K, i, j, k, idx));
+    const double m = sq(ax) + sq(ay) + sq(az);
+    return (T)(-m * grad_hat<T, c>(L, G, K, i, j, k, idx));
   } else if constexpr (LD == LD_SH) {
-    const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);
+    const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
     const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
     return (T)(-alin * (double)ld<T>(L.a[0], idx));
   } else if constexpr (LD == LD_RHOSTAR) {
@@ -388,7 +387,7 @@ __device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Kv& K, Ix i
     in.pho = ld<double>(S.d[0], idx);
     in.nold = (double)ld<T>(S.d[1], idx);
   } else if constexpr (ST == ST_FILTER) {
-    in.ph = K.xi * K.fy * K.fz;  // i-table f_x
+    in.ph = K.fx * K.fy * K.fz;
   }
   return in;
 }
@@ -420,7 +419,7 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
     if (z && S.dc_out) S.dc_out[0] = r;
     return;
   }
-  const double mq2 = sq(K.xi) + sq(K.qy) + sq(K.qz);  // i-table q_x
+  const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
   const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
   const double nh = (D.has_dc & z) ? 8.0 * D.dc : (double)v;
   const double nold = S.first ? nh : ((D.has_old & z) ? D.old : in.nold);

@@ -232,56 +232,145 @@ struct TileGeo {
   }
 };
 
-// One workgroup's work: tile `tile` of the pass described by A.
+// LDS of a pass: the tile, the twiddles and the pass-axis wavenumber tables
+// of mask TAB (bit 0: q, 1: s, 2: f; each N doubles, in that order) with, in
+// z passes, the tile's per-row q_x, s_x, q_y, s_y.  It is declared once per
+// kernel, so the passes of a group kernel share it.
+template <typename T, int N, int AX, bool INV, int V, int TAB>
+struct Smem {
+  using C = TileCfg<T, N, AX, V>;
+  static constexpr int M = C::M, TK = C::TK, TKP = C::TKP;
+  static constexpr int NTP = INV ? TwLayout<M>::NPRE : TwLayout<M>::NPOST;
+  static constexpr int tab = TAB, ntab = kv_count(TAB);
+  static constexpr long bytes() {
+    return (long)sizeof(cx<T>) * ((long)M * TKP + M + NTP) +
+           (long)sizeof(double) * (ntab * N + ((AX == 2 && ntab > 0) ? 4 * TK : 0));
+  }
+  cx<T> buf[M * TKP];  // the tile: complex slot m of pencil p at m * TKP + p
+  cx<T> ts[M];         // stage twiddles
+  cx<T> tp[NTP];       // pre- (inverse) or post-processing (forward) twiddles
+  double xt[ntab > 0 ? ntab * N : 1];
+  double rowt[(AX == 2 && ntab > 0) ? 4 * TK : 1];
+};
+
+// Tables staged in LDS: those of mask NT where they fit beside the tile while
+// two workgroups share a CU (160 KiB); otherwise none, and the operator reads
+// them from global memory.
+template <typename T, int N, int AX, bool INV, int V, int NT>
+constexpr int smem_tables() {
+  return (NT != 0 && Smem<T, N, AX, INV, V, NT>::bytes() <= 81920) ? NT : 0;
+}
+
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
-__device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
+using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<AX, LD, ST>()>()>;
+
+// One workgroup's work: tile `tile` of the pass described by A, in the LDS
+// sm.  stage = false skips the staging of twiddles and tables a previous
+// pass of the same kernel (same axis and direction) left in sm.
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM>
+__device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM& sm,
+                                          const bool stage = true) {
   using C = TileCfg<T, N, AX, V>;
   using PL = Plan<N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1, NB1 = E / R1;
   using TW = TwLayout<M>;
   constexpr int NTP = INV ? TW::NPRE : TW::NPOST;
-  __shared__ cx<T> buf[M * TKP];
-  __shared__ cx<T> ts[M];     // stage twiddles
-  __shared__ cx<T> tp[NTP];   // pre- (inverse) or post-processing (forward) twiddles
+  static_assert(SM::NTP == NTP && SM::TKP == TKP, "LDS layout of another pass");
+  cx<T>* const buf = sm.buf;
+  cx<T>* const ts = sm.ts;
+  cx<T>* const tp = sm.tp;
   T* rbuf = (T*)buf;  // real view: sample n of pencil p at ((n>>1)*TKP + p)*2 + (n&1)
   const Geo G = A.geo;
   const int tid = threadIdx.x;
   const bool dst = A.dst != 0;
 
-  // The x-pass spectral operators read one i-indexed wavenumber table per
-  // launch (itab); it is staged in LDS where two workgroups per CU still fit
-  // (160 KiB), else read from global memory per element.
+  // Wavenumbers.  The pass-axis tables (q and s, or the filter factors f)
+  // are staged in LDS where they fit (SM::ntab), else read per element; the
+  // other axes' values are fixed per thread (strided passes: one i or j per
+  // tile, one column k per thread) or per row (z passes: staged per row).
+  constexpr int NT = kv_tables<AX, LD, ST>();
   constexpr bool KV = uses_kv<LD, ST>();
-  static_assert(!KV || AX == kAxX, "wavenumber operators run in x passes");
-  constexpr bool XT_LDS =
-      KV && sizeof(cx<T>) * ((long)M * TKP + M + NTP) + sizeof(double) * N <= 81920;
-  __shared__ double xt[XT_LDS ? N : 1];
-  const double* xsrc = nullptr;
-  if constexpr (KV) xsrc = itab<LD, ST>(A.ld, A.waves);
-  {
-    const cx<T>* __restrict__ tg = (const cx<T>*)A.tw;
-    for (int u = tid; u < M; u += BLOCK) ts[u] = tg[TW::STAGE + u];
-    for (int u = tid; u < NTP; u += BLOCK) tp[u] = tg[(INV ? TW::PRE : TW::POST) + u];
-    if constexpr (XT_LDS) {
-      for (int u = tid; u < N; u += BLOCK) xt[u] = xsrc[u];
-      __syncthreads();
-    }
-  }
+  constexpr bool XT_LDS = NT != 0 && (SM::tab & NT) == NT;
+  // slot of table bit b in sm.xt
+  constexpr int SQ = 0, SS = (SM::tab & 1) ? 1 : 0, SF = ((SM::tab & 1) ? 1 : 0) + ((SM::tab & 2) ? 1 : 0);
+  const Waves& W = A.waves;
+  const double* tq = AX == 0 ? W.qx : (AX == 1 ? W.qy : W.qz);
+  const double* tsn = AX == 0 ? W.sx : (AX == 1 ? W.sy : W.sz);
+  const double* tf = AX == 0 ? W.fx : (AX == 1 ? W.fy : W.fz);
 
   TileGeo<AX, N, TK, BLOCK> tg;
   tg.init(G, tile, tid);
 
+  if (stage) {
+    const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
+    for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
+    for (int u = tid; u < NTP; u += BLOCK) tp[u] = tw[(INV ? TW::PRE : TW::POST) + u];
+    if constexpr (XT_LDS) {
+      for (int u = tid; u < N; u += BLOCK) {
+        if constexpr ((SM::tab & 1) != 0) sm.xt[SQ * N + u] = tq[u];
+        if constexpr ((SM::tab & 2) != 0) sm.xt[SS * N + u] = tsn[u];
+        if constexpr ((SM::tab & 4) != 0) sm.xt[SF * N + u] = tf[u];
+      }
+      if constexpr (AX == kAxZ) {
+        for (int r = tid; r < TK; r += BLOCK) {
+          int row = tg.row0 + r;
+          if (row >= tg.nrows) row = tg.row0;
+          const int i = row / G.ny, j = row - i * G.ny;
+          sm.rowt[4 * r + 0] = W.qx[i];
+          sm.rowt[4 * r + 1] = W.sx[i];
+          sm.rowt[4 * r + 2] = W.qy[j];
+          sm.rowt[4 * r + 3] = W.sy[j];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
   // FFT-phase thread coordinates: pencil p, thread t within the pencil
   const int p = tid % TK, t = tid / TK;
-  // wavenumbers of this thread's (j, k); an x-pass thread keeps its column
-  // p = tid % TK in every phase
+  // wavenumbers fixed per thread in strided passes (a thread keeps its
+  // column p = tid % TK in every phase)
   Kv kv{};
-  if constexpr (KV) kv = kv_thread(A.waves, tg.t_j, tg.kc);
-  auto kv_at = [&](int i) {
+  if constexpr (KV && AX == kAxX) {
+    const int j = tg.t_j, k = tg.kc;
+    kv.qy = W.qy[j]; kv.sy = W.sy[j]; kv.fy = W.fy[j];
+    kv.qz = W.qz[k]; kv.sz = W.sz[k]; kv.fz = W.fz[k];
+  } else if constexpr (KV && AX == kAxY) {
+    const int i = tg.t_i, k = tg.kc;
+    kv.qx = W.qx[i]; kv.sx = W.sx[i]; kv.fx = W.fx[i];
+    kv.qz = W.qz[k]; kv.sz = W.sz[k]; kv.fz = W.fz[k];
+  }
+  // element (i, j, k) of pencil q
+  auto kv_at = [&](int i, int j, int k, int q) {
     Kv K = kv;
-    if constexpr (XT_LDS) K.xi = xt[i];
-    else if constexpr (KV) K.xi = xsrc[i];
+    if constexpr (KV) {
+      const int m = AX == 0 ? i : (AX == 1 ? j : k);
+      double a = 0.0, b = 0.0, f = 0.0;  // pass-axis q, s, f
+      if constexpr (XT_LDS) {
+        if constexpr ((NT & 1) != 0) a = sm.xt[SQ * N + m];
+        if constexpr ((NT & 2) != 0) b = sm.xt[SS * N + m];
+        if constexpr ((NT & 4) != 0) f = sm.xt[SF * N + m];
+      } else {
+        if constexpr ((NT & 1) != 0) a = tq[m];
+        if constexpr ((NT & 2) != 0) b = tsn[m];
+        if constexpr ((NT & 4) != 0) f = tf[m];
+      }
+      if constexpr (AX == kAxX) {
+        K.qx = a; K.sx = b; K.fx = f;
+      } else if constexpr (AX == kAxY) {
+        K.qy = a; K.sy = b; K.fy = f;
+      } else {
+        K.qz = a; K.sz = b; K.fz = f;
+        if constexpr (XT_LDS) {
+          K.qx = sm.rowt[4 * q + 0]; K.sx = sm.rowt[4 * q + 1];
+          K.qy = sm.rowt[4 * q + 2]; K.sy = sm.rowt[4 * q + 3];
+        } else {
+          K.qx = W.qx[i]; K.sx = W.sx[i]; K.qy = W.qy[j]; K.sy = W.sy[j];
+        }
+        K.fx = W.fx[i]; K.fy = W.fy[j];
+      }
+    }
     return K;
   };
   // FP64 sum of the loaded values (mixed-precision DC modes): the forward
@@ -309,7 +398,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -411,7 +500,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          in[c] = store_inputs<T, ST>(A.st, kv_at(i), idx);
+          in[c] = store_inputs<T, ST>(A.st, kv_at(i, j, k, q), idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -422,7 +511,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
           if (!ok) continue;
           bool oddn;
           const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
-          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i), idx, y, in[c], dcv);
+          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i, j, k, q), idx, y, in[c], dcv);
           else store_eval<T, ST>(A.st, idx, y);
         }
       }
@@ -455,7 +544,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = finish<T, LD>(raw[c], A.ld, A.phys, kv_at(i), G, i, j, k);
+            xs[c] = finish<T, LD>(raw[c], A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k);
           }
         } else {
 #pragma unroll
@@ -464,7 +553,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile) {
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i), G, i, j, k, idx);
+            xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k, idx);
           }
         }
 #pragma unroll
@@ -603,8 +692,9 @@ __device__ __forceinline__ int tile_order(const Geo& g, int b, int nb) {
 
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const PassArgs A) {
+  __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
   pass_tile<T, N, AX, INV, LD, ST, V>(
-      A, tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x));
+      A, tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
@@ -619,7 +709,8 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
     const int mid = (lo + hi) >> 1;
     if (prefix[mid] <= b) lo = mid; else hi = mid;
   }
-  pass_tile<T, N, AX, INV, LD, ST, V>(descs[lo], b - prefix[lo]);
+  __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
+  pass_tile<T, N, AX, INV, LD, ST, V>(descs[lo], b - prefix[lo], sm);
 }
 
 
@@ -686,7 +777,8 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
     if (inv && st == ST_PLAIN) {
       switch (ld) {
 #define QSM_L(OP) case OP: return launch_one<T, N, AX, true, OP, ST_PLAIN>(A, nb, s, B);
-        QSM_L(LD_COEF_F) QSM_L(LD_COEF_G) QSM_L(LD_GRAD) QSM_L(LD_LAPGRAD) QSM_L(LD_SH) QSM_L(LD_PRESSURE) QSM_L(LD_VEL0) QSM_L(LD_VEL1)
+        QSM_L(LD_COEF_F) QSM_L(LD_COEF_G) QSM_L(LD_GRAD_X) QSM_L(LD_GRAD_Y) QSM_L(LD_GRAD_Z)
+        QSM_L(LD_LAPGRAD_X) QSM_L(LD_LAPGRAD_Y) QSM_L(LD_LAPGRAD_Z) QSM_L(LD_SH) QSM_L(LD_PRESSURE) QSM_L(LD_VEL0) QSM_L(LD_VEL1)
         QSM_L(LD_VEL2) QSM_L(LD_SCALE)
 #undef QSM_L
         default: break;
