@@ -146,6 +146,9 @@ struct Timer {
   }
 };
 
+// timer categories of the group kernels: load code kGroupCode0 + group
+constexpr int kGroupCode0 = 40;
+
 int timer_key(int axis, bool inv, int ld, int st, int v) {
   return (((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st) * 2 + v;
 }
@@ -433,6 +436,69 @@ int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreO
   return run_pass(h, 0, false, dst_axis == 0, plain_load(tmp), S, false, nullptr);
 }
 
+// ---- group transforms: several 3D transforms whose first pass reads the
+// same fields run that pass as one kernel (launch_group), then their other
+// passes one by one.  Inverse groups run x first (x, y, z, as inv3), forward
+// groups z first (z, y, x, as fwd3).  Used by single handles; an ensemble
+// records its passes one transform at a time and keeps the per-transform path.
+struct GroupItem {
+  LoadOp L;
+  void* out;     // output field; the group pass writes it, the others run in place
+  int dst_axis;  // axis of the DST (-1: DCT on every axis)
+};
+
+// launch of the first pass of a group, with the timing of one pass; `reads`
+// is the number of distinct fields the group reads (algorithmic bytes)
+int run_group_first(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
+  const int axis = group_axis(group);
+  GroupArgs G{};
+  G.geo = h->geo;
+  G.waves = h->waves;
+  G.phys = h->phys;
+  G.tw = h->tw[axis];
+  for (int o = 0; o < count; ++o) {
+    G.out[o].ld = it[o].L;
+    G.out[o].st = plain_store(it[o].out);
+    G.out[o].dst = it[o].dst_axis == axis ? 1 : 0;
+  }
+  if (h->dry) return QSM_OK;
+  if (h->rec) return fail(QSM_ERR_STATE, "group passes cannot be recorded");
+  const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
+  Timer& T = h->timer;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+  hipError_t e = launch_group(h->fp32, n, group, G, nullptr, h->stream);
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "group %d launch: %s", group, hipGetErrorString(e));
+  if (T.on) {
+    hipEventRecord(eb, h->stream);
+    T.recs.push_back({ea, eb, timer_key(axis, group_inverse(group), kGroupCode0 + group, ST_PLAIN, 0),
+                      (double)(reads + count) * (double)h->n * (double)h->esz});
+  }
+  return QSM_OK;
+}
+
+// inverse transforms x, y, z (the x pass as one group kernel)
+int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
+  int rc = run_group_first(h, group, it, count, reads);
+  for (int o = 0; o < count && !rc; ++o) {
+    rc = run_pass(h, 1, true, it[o].dst_axis == 1, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
+    if (!rc)
+      rc = run_pass(h, 2, true, it[o].dst_axis == 2, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
+  }
+  return rc;
+}
+
+// forward transforms z, y, x with plain stores (the z pass as one group kernel)
+int fwd3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
+  int rc = run_group_first(h, group, it, count, reads);
+  for (int o = 0; o < count && !rc; ++o) {
+    rc = run_pass(h, 1, false, it[o].dst_axis == 1, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
+    if (!rc)
+      rc = run_pass(h, 0, false, it[o].dst_axis == 0, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
+  }
+  return rc;
+}
+
 void* F(qsm_handle_s* h, int f) { return h->f[f]; }
 
 // psihat in the field precision: the FP64 state, or its FP32 copy in mixed
@@ -452,21 +518,47 @@ int quasi_step(qsm_handle_s* h) {
   const bool dlive = n > p.divv_switch;  // D is computed from this step on
   h->divv_valid = false;
 
-  // S1: F, G (quasi.cpp:1034-1057)
+  const bool grp = !h->rec;  // group kernels (single handles)
   LoadOp L{};
-  L.code = LD_COEF_F; L.a[0] = psih;
-  if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
-  L.code = LD_COEF_G;
-  if ((rc = inv3(h, -1, L, F(h, F_G)))) return rc;
+  if (grp) {
+    // S1 + S2 (quasi.cpp:1034-1232): F, G, grad psi, laplacian of grad psi
+    // from psihat in one group; grad rho (lagged) from rhohat in another
+    GroupItem it[8];
+    it[0] = {LoadOp{}, F(h, F_F), -1};
+    it[0].L.code = LD_COEF_F;
+    it[1] = {LoadOp{}, F(h, F_G), -1};
+    it[1].L.code = LD_COEF_G;
+    for (int c = 0; c < 3; ++c) {
+      it[2 + c] = {LoadOp{}, F(h, F_GX + c), c};
+      it[2 + c].L.code = LD_GRAD_X + c;
+      it[5 + c] = {LoadOp{}, F(h, F_LX + c), c};
+      it[5 + c].L.code = LD_LAPGRAD_X + c;
+    }
+    for (auto& x : it) { x.L.a[0] = psih; x.L.axis = x.dst_axis; }
+    if ((rc = inv3_group(h, GRP_PSI, it, 8, 1))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      it[c] = {LoadOp{}, F(h, F_RX + c), c};
+      it[c].L.code = LD_GRAD_X + c;
+      it[c].L.axis = c;
+      it[c].L.a[0] = F(h, F_RHOHAT);
+    }
+    if ((rc = inv3_group(h, GRP_RHO, it, 3, 1))) return rc;
+  } else {
+    // S1: F, G (quasi.cpp:1034-1057)
+    L.code = LD_COEF_F; L.a[0] = psih;
+    if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
+    L.code = LD_COEF_G;
+    if ((rc = inv3(h, -1, L, F(h, F_G)))) return rc;
 
-  // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
-    if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
-    if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
-    L = LoadOp{}; L.code = LD_LAPGRAD_X + c; L.axis = c; L.a[0] = psih;
-    if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
+    // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
+      if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
+      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
+      if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
+      L = LoadOp{}; L.code = LD_LAPGRAD_X + c; L.axis = c; L.a[0] = psih;
+      if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
+    }
   }
 
   // S3: density law + filter; rho (quasi.cpp:1237-1268)
@@ -489,24 +581,47 @@ int quasi_step(qsm_handle_s* h) {
   }
 
   // S5: stress (quasi.cpp:1298-1330)
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
-    L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F); L.a[3] = F(h, F_GX + c); L.a[4] = F(h, F_LX + c);
-    if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+  if (grp) {
+    GroupItem it[3];
+    for (int c = 0; c < 3; ++c) {
+      it[c] = {LoadOp{}, F(h, F_SX + c), c};
+      it[c].L.code = LD_STRESS; it[c].L.axis = c;
+      it[c].L.a[0] = rho; it[c].L.a[1] = F(h, F_G); it[c].L.a[2] = F(h, F_F);
+      it[c].L.a[3] = F(h, F_GX + c); it[c].L.a[4] = F(h, F_LX + c);
+    }
+    if ((rc = fwd3_group(h, GRP_STRESS, it, 3, 9))) return rc;
+  } else {
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
+      L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F); L.a[3] = F(h, F_GX + c); L.a[4] = F(h, F_LX + c);
+      if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+    }
   }
 
-  // S6: pressure (quasi.cpp:1336-1418)
-  L = LoadOp{}; L.code = LD_PRESSURE;
-  L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
-  if ((rc = inv3(h, -1, L, F(h, F_P)))) return rc;
-
-  // S7: velocity projection (quasi.cpp:1425-1884)
+  // S6 + S7: pressure and velocity projection (quasi.cpp:1336-1884)
   const int n0 = p.compat_ranks > 0 ? p.nx / p.compat_ranks : 0;
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
+  if (grp) {
+    GroupItem it[4];
+    it[0] = {LoadOp{}, F(h, F_P), -1};
+    it[0].L.code = LD_PRESSURE;
+    for (int c = 0; c < 3; ++c) {
+      it[1 + c] = {LoadOp{}, F(h, F_VX + c), c};
+      it[1 + c].L.code = LD_VEL0 + c;
+      it[1 + c].L.n0 = n0;
+      it[1 + c].L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
+    }
+    for (auto& x : it) { x.L.a[0] = F(h, F_SX); x.L.a[1] = F(h, F_SY); x.L.a[2] = F(h, F_SZ); }
+    if ((rc = inv3_group(h, GRP_PROJ, it, 4, dlive ? 4 : 3))) return rc;
+  } else {
+    L = LoadOp{}; L.code = LD_PRESSURE;
     L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
-    L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
-    if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+    if ((rc = inv3(h, -1, L, F(h, F_P)))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
+      L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
+      L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
+      if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+    }
   }
 
   // S8: the D output, IDCT(scale D-hat) (quasi.cpp:1888-1900), is read by no
@@ -550,22 +665,48 @@ int stb_step(qsm_handle_s* h) {
   // sh = IDCT(-aLin psihat) (quasi-stb.cpp:1015-1038), kept in F
   LoadOp L{};
   const void* psih = psihat_src(h);
-  L.code = LD_SH; L.a[0] = psih;
-  if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
-    if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-  }
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
-    L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);
-    if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
-  }
   const int n0 = h->p.compat_ranks > 0 ? h->p.nx / h->p.compat_ranks : 0;
-  for (int c = 0; c < 3; ++c) {
-    L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
-    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = nullptr;
-    if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+  if (!h->rec) {  // group kernels (single handles)
+    GroupItem it[4];
+    it[0] = {LoadOp{}, F(h, F_F), -1};
+    it[0].L.code = LD_SH;
+    for (int c = 0; c < 3; ++c) {
+      it[1 + c] = {LoadOp{}, F(h, F_GX + c), c};
+      it[1 + c].L.code = LD_GRAD_X + c;
+      it[1 + c].L.axis = c;
+    }
+    for (auto& x : it) x.L.a[0] = psih;
+    if ((rc = inv3_group(h, GRP_STB_PSI, it, 4, 1))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      it[c] = {LoadOp{}, F(h, F_SX + c), c};
+      it[c].L.code = LD_STRESS_STB; it[c].L.axis = c;
+      it[c].L.a[0] = F(h, F_F); it[c].L.a[1] = psi; it[c].L.a[2] = F(h, F_GX + c);
+    }
+    if ((rc = fwd3_group(h, GRP_STB_STRESS, it, 3, 5))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      it[c] = {LoadOp{}, F(h, F_VX + c), c};
+      it[c].L.code = LD_VEL0 + c; it[c].L.n0 = n0;
+      it[c].L.a[0] = F(h, F_SX); it[c].L.a[1] = F(h, F_SY); it[c].L.a[2] = F(h, F_SZ);
+      it[c].L.a[3] = nullptr;
+    }
+    if ((rc = inv3_group(h, GRP_STB_PROJ, it, 3, 3))) return rc;
+  } else {
+    L.code = LD_SH; L.a[0] = psih;
+    if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
+      if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
+    }
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
+      L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);
+      if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+    }
+    for (int c = 0; c < 3; ++c) {
+      L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
+      L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ); L.a[3] = nullptr;
+      if ((rc = inv3(h, c, L, F(h, F_VX + c)))) return rc;
+    }
   }
   L = LoadOp{}; L.code = h->p.hydro == 1 ? LD_NONLIN_STB : LD_NONLIN_STB0;
   L.a[0] = psi; L.a[1] = F(h, F_VX); L.a[2] = F(h, F_VY); L.a[3] = F(h, F_VZ);

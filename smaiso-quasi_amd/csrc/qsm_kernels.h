@@ -54,6 +54,42 @@ bool step_supported(int n);
 int pass_tile_k(int prec_fp32, int n, int axis, int variant = 0);
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 
+// Group passes.  One kernel runs the first pass of several transforms that
+// read the same fields: each workgroup runs them one after another on its
+// tile, so the shared inputs come from HBM once and from L2 after that.
+// Inverse groups run the x pass of x, y, z inverse transforms, forward groups
+// the z pass of z, y, x forward transforms (group_axis).
+enum GroupCode : int {
+  GRP_PSI = 0,     // inverse: F, G, grad psi (x, y, z), lap grad psi (x, y, z) from psihat
+  GRP_RHO,         // inverse: grad rho (x, y, z) from rhohat
+  GRP_PROJ,        // inverse: p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
+  GRP_STRESS,      // forward: S_x, S_y, S_z
+  GRP_STB_PSI,     // inverse: sh, grad psi (x, y, z)
+  GRP_STB_PROJ,    // inverse: v_x, v_y, v_z
+  GRP_STB_STRESS,  // forward: S_x, S_y, S_z of the stb step
+  GRP_COUNT
+};
+constexpr int kMaxGroup = 8;
+struct GroupOut {
+  LoadOp ld;   // the output's load operator (its code must match the group's list)
+  StoreOp st;  // plain store into the output field
+  int dst;     // 1: DST along the group's axis for this output
+  int pad_;
+};
+struct GroupArgs {
+  Geo geo;
+  Waves waves;
+  Phys phys;
+  const void* tw;  // twiddle table of the group's axis
+  GroupOut out[kMaxGroup];
+};
+// operators of a group in output order, their count, and the direction
+int group_ops(int group, int* codes);
+bool group_inverse(int group);
+int group_axis(int group);  // 0 (x) or 2 (z)
+hipError_t launch_group(int prec_fp32, int n, int group, const GroupArgs& G, long* nblocks,
+                        hipStream_t s);
+
 // FP64 compensated reductions
 hipError_t launch_diag(int prec_fp32, const void* psi, const void* pold, const void* rho,
                        const void* F, long n, const Phys& P, double* part, int nblocks,

@@ -18,6 +18,60 @@ QSM_DECL(double, 1)
 QSM_DECL(double, 2)
 #undef QSM_DECL
 
+hipError_t launch_group_float_0(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
+hipError_t launch_group_float_2(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
+hipError_t launch_group_double_0(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
+hipError_t launch_group_double_2(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
+
+hipError_t launch_group(int prec_fp32, int n, int group, const GroupArgs& G, long* nblocks,
+                        hipStream_t s) {
+  if (group_axis(group) == kAxX)
+    return prec_fp32 ? launch_group_float_0(n, group, G, nblocks, s)
+                     : launch_group_double_0(n, group, G, nblocks, s);
+  return prec_fp32 ? launch_group_float_2(n, group, G, nblocks, s)
+                   : launch_group_double_2(n, group, G, nblocks, s);
+}
+
+template <int... LDs>
+static int list_codes(OpList<LDs...>, int* codes) {
+  int k = 0;
+  ((codes ? (void)(codes[k] = LDs) : (void)0, ++k), ...);
+  return k;
+}
+
+int group_ops(int group, int* codes) {
+  switch (group) {
+    case GRP_PSI: return list_codes(GroupOps<GRP_PSI>::L{}, codes);
+    case GRP_RHO: return list_codes(GroupOps<GRP_RHO>::L{}, codes);
+    case GRP_PROJ: return list_codes(GroupOps<GRP_PROJ>::L{}, codes);
+    case GRP_STRESS: return list_codes(GroupOps<GRP_STRESS>::L{}, codes);
+    case GRP_STB_PSI: return list_codes(GroupOps<GRP_STB_PSI>::L{}, codes);
+    case GRP_STB_PROJ: return list_codes(GroupOps<GRP_STB_PROJ>::L{}, codes);
+    case GRP_STB_STRESS: return list_codes(GroupOps<GRP_STB_STRESS>::L{}, codes);
+    default: return 0;
+  }
+}
+
+bool group_inverse(int group) {
+  switch (group) {
+#define QSM_G(GG) case GG: return GroupOps<GG>::INV;
+    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STRESS) QSM_G(GRP_STB_PSI)
+    QSM_G(GRP_STB_PROJ) QSM_G(GRP_STB_STRESS)
+#undef QSM_G
+    default: return false;
+  }
+}
+
+int group_axis(int group) {
+  switch (group) {
+#define QSM_G(GG) case GG: return GroupOps<GG>::AX;
+    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STRESS) QSM_G(GRP_STB_PSI)
+    QSM_G(GRP_STB_PROJ) QSM_G(GRP_STB_STRESS)
+#undef QSM_G
+    default: return -1;
+  }
+}
+
 hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
                        long* nb, hipStream_t s, const BatchLaunch* B, int variant) {
   const int v = (ld == LD_PLAIN && st == ST_PLAIN) ? variant : 0;

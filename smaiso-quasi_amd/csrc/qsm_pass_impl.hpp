@@ -242,15 +242,17 @@ struct Smem {
   static constexpr int M = C::M, TK = C::TK, TKP = C::TKP;
   static constexpr int NTP = INV ? TwLayout<M>::NPRE : TwLayout<M>::NPOST;
   static constexpr int tab = TAB, ntab = kv_count(TAB);
+  static constexpr int nrowt = (AX == 2 && ntab > 0) ? 4 * TK : 0;
   static constexpr long bytes() {
     return (long)sizeof(cx<T>) * ((long)M * TKP + M + NTP) +
-           (long)sizeof(double) * (ntab * N + ((AX == 2 && ntab > 0) ? 4 * TK : 0));
+           (long)sizeof(double) * (ntab * N + nrowt);
   }
   cx<T> buf[M * TKP];  // the tile: complex slot m of pencil p at m * TKP + p
   cx<T> ts[M];         // stage twiddles
   cx<T> tp[NTP];       // pre- (inverse) or post-processing (forward) twiddles
-  double xt[ntab > 0 ? ntab * N : 1];
-  double rowt[(AX == 2 && ntab > 0) ? 4 * TK : 1];
+  // the tables (ntab * N values), then the per-row values (nrowt); one array,
+  // so a pass without tables adds 8 bytes, not 16
+  double xt[(ntab * N + nrowt) > 0 ? ntab * N + nrowt : 1];
 };
 
 // Tables staged in LDS: those of mask NT where they fit beside the tile while
@@ -268,8 +270,11 @@ using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<A
 // sm.  stage = false skips the staging of twiddles and tables a previous
 // pass of the same kernel (same axis and direction) left in sm.
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM>
-__device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM& sm,
-                                          const bool stage = true) {
+__device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Phys& Ph,
+                                          const LoadOp& Lop, const StoreOp& Sop,
+                                          const void* twp, double* part, const int dstf,
+                                          const int tile, SM& sm, const bool stage = true,
+                                          const int tid = (int)threadIdx.x) {
   using C = TileCfg<T, N, AX, V>;
   using PL = Plan<N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
@@ -281,9 +286,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
   cx<T>* const ts = sm.ts;
   cx<T>* const tp = sm.tp;
   T* rbuf = (T*)buf;  // real view: sample n of pencil p at ((n>>1)*TKP + p)*2 + (n&1)
-  const Geo G = A.geo;
-  const int tid = threadIdx.x;
-  const bool dst = A.dst != 0;
+  const bool dst = dstf != 0;
 
   // Wavenumbers.  The pass-axis tables (q and s, or the filter factors f)
   // are staged in LDS where they fit (SM::ntab), else read per element; the
@@ -292,9 +295,9 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
   constexpr int NT = kv_tables<AX, LD, ST>();
   constexpr bool KV = uses_kv<LD, ST>();
   constexpr bool XT_LDS = NT != 0 && (SM::tab & NT) == NT;
+  constexpr bool ROWT = AX == kAxZ && SM::ntab > 0;  // per-row wavenumbers staged (z)
   // slot of table bit b in sm.xt
   constexpr int SQ = 0, SS = (SM::tab & 1) ? 1 : 0, SF = ((SM::tab & 1) ? 1 : 0) + ((SM::tab & 2) ? 1 : 0);
-  const Waves& W = A.waves;
   const double* tq = AX == 0 ? W.qx : (AX == 1 ? W.qy : W.qz);
   const double* tsn = AX == 0 ? W.sx : (AX == 1 ? W.sy : W.sz);
   const double* tf = AX == 0 ? W.fx : (AX == 1 ? W.fy : W.fz);
@@ -303,24 +306,24 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
   tg.init(G, tile, tid);
 
   if (stage) {
-    const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
+    const cx<T>* __restrict__ tw = (const cx<T>*)twp;
     for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
     for (int u = tid; u < NTP; u += BLOCK) tp[u] = tw[(INV ? TW::PRE : TW::POST) + u];
-    if constexpr (XT_LDS) {
+    if constexpr (SM::ntab > 0) {
       for (int u = tid; u < N; u += BLOCK) {
         if constexpr ((SM::tab & 1) != 0) sm.xt[SQ * N + u] = tq[u];
         if constexpr ((SM::tab & 2) != 0) sm.xt[SS * N + u] = tsn[u];
         if constexpr ((SM::tab & 4) != 0) sm.xt[SF * N + u] = tf[u];
       }
-      if constexpr (AX == kAxZ) {
+      if constexpr (ROWT) {
         for (int r = tid; r < TK; r += BLOCK) {
           int row = tg.row0 + r;
           if (row >= tg.nrows) row = tg.row0;
           const int i = row / G.ny, j = row - i * G.ny;
-          sm.rowt[4 * r + 0] = W.qx[i];
-          sm.rowt[4 * r + 1] = W.sx[i];
-          sm.rowt[4 * r + 2] = W.qy[j];
-          sm.rowt[4 * r + 3] = W.sy[j];
+          sm.xt[SM::ntab * N + 4 * r + 0] = W.qx[i];
+          sm.xt[SM::ntab * N + 4 * r + 1] = W.sx[i];
+          sm.xt[SM::ntab * N + 4 * r + 2] = W.qy[j];
+          sm.xt[SM::ntab * N + 4 * r + 3] = W.sy[j];
         }
       }
       __syncthreads();
@@ -362,9 +365,9 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
         K.qy = a; K.sy = b; K.fy = f;
       } else {
         K.qz = a; K.sz = b; K.fz = f;
-        if constexpr (XT_LDS) {
-          K.qx = sm.rowt[4 * q + 0]; K.sx = sm.rowt[4 * q + 1];
-          K.qy = sm.rowt[4 * q + 2]; K.sy = sm.rowt[4 * q + 3];
+        if constexpr (ROWT) {
+          K.qx = sm.xt[SM::ntab * N + 4 * q + 0]; K.sx = sm.xt[SM::ntab * N + 4 * q + 1];
+          K.qy = sm.xt[SM::ntab * N + 4 * q + 2]; K.sy = sm.xt[SM::ntab * N + 4 * q + 3];
         } else {
           K.qx = W.qx[i]; K.sx = W.sx[i]; K.qy = W.qy[j]; K.sy = W.sy[j];
         }
@@ -398,7 +401,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k, idx);
+          xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -446,7 +449,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
         int i, j, k;
         Ix idx;
         tg.at(n, i, j, k, idx);
-        if (tg.colok) store_eval<T, ST>(A.st, idx, val);
+        if (tg.colok) store_eval<T, ST>(Sop, idx, val);
       }
     };
     // outputs o = kk, N-kk, M-kk, M+kk of item kk = t + C are t + K or K - t
@@ -467,7 +470,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
       } else {
         const Ix ia = (up ? gu : gd) + (Ix)K * tg.axs;
         const Ix ib = (up ? gd : gu) + (Ix)(N - 1 - K) * tg.axs;
-        if (tg.colok) store_eval<T, ST>(A.st, dst ? ib : ia, val);
+        if (tg.colok) store_eval<T, ST>(Sop, dst ? ib : ia, val);
       }
     };
     if constexpr (STAGED) __syncthreads();
@@ -488,7 +491,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = store_reads<ST>() ? (PER < 8 ? PER : 8) : PER;
       DcIn dcv{};
-      if constexpr (store_reads<ST>()) dcv = dc_in(A.st);
+      if constexpr (store_reads<ST>()) dcv = dc_in(Sop);
       WalkPos<AX, N, TK, TKP, BLOCK, 0> wp;
       wp.init(tid);
 #pragma unroll 1
@@ -500,7 +503,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          in[c] = store_inputs<T, ST>(A.st, kv_at(i, j, k, q), idx);
+          in[c] = store_inputs<T, ST>(Sop, kv_at(i, j, k, q), idx);
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -511,8 +514,8 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
           if (!ok) continue;
           bool oddn;
           const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
-          if constexpr (store_reads<ST>()) store_apply<T, ST>(A.st, A.phys, kv_at(i, j, k, q), idx, y, in[c], dcv);
-          else store_eval<T, ST>(A.st, idx, y);
+          if constexpr (store_reads<ST>()) store_apply<T, ST>(Sop, Ph, kv_at(i, j, k, q), idx, y, in[c], dcv);
+          else store_eval<T, ST>(Sop, idx, y);
         }
       }
     }
@@ -536,7 +539,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            raw[c] = fetch<T, LD>(A.ld, G, i, j, k, idx);
+            raw[c] = fetch<T, LD>(Lop, G, i, j, k, idx);
           }
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
@@ -544,7 +547,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = finish<T, LD>(raw[c], A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k);
+            xs[c] = finish<T, LD>(raw[c], Lop, Ph, kv_at(i, j, k, q), G, i, j, k);
           }
         } else {
 #pragma unroll
@@ -553,7 +556,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = load_eval<T, LD>(A.ld, A.phys, kv_at(i, j, k, q), G, i, j, k, idx);
+            xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
           }
         }
 #pragma unroll
@@ -609,7 +612,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
         if (!ok) continue;
         T y = rbuf[wp.pos(tid, s, oddn)];
         if (dst & oddn) y = -y;
-        store_eval<T, ST>(A.st, idx, y);
+        store_eval<T, ST>(Sop, idx, y);
       }
     } else {
       run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, v, buf, p, t, ts);
@@ -633,7 +636,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
               const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
               // constant when the thread range t < TP stays on one side of M
               const bool low = (2 * (TP - 1) + u0 < M) ? true : (u0 >= M ? false : 2 * t + u0 < M);
-              store_eval<T, ST>(A.st, low ? iu : idn, (!low && dst) ? -y : y);
+              store_eval<T, ST>(Sop, low ? iu : idn, (!low && dst) ? -y : y);
             }
           }
         }
@@ -643,7 +646,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
 
   // ---- FP64 block sum of the loaded values (deterministic order)
   if constexpr (DSUM) {
-    if (!A.part) return;
+    if (!part) return;
     __syncthreads();
     double* red = (double*)buf;
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, 64);
@@ -652,7 +655,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs& A, const int tile, SM&
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < (BLOCK + 63) / 64; ++w) s += red[w];
-      A.part[tile] = s;
+      part[tile] = s;
     }
   }
 }
@@ -694,7 +697,113 @@ template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const PassArgs A) {
   __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
   pass_tile<T, N, AX, INV, LD, ST, V>(
-      A, tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
+      A.geo, A.waves, A.phys, A.ld, A.st, A.tw, A.part, A.dst,
+      tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
+}
+
+// ---- group passes (z axis): the operator list of each group
+template <int... LDs>
+struct OpList {};
+template <int G>
+struct GroupOps;
+template <>
+struct GroupOps<GRP_PSI> {
+  using L = OpList<LD_COEF_F, LD_COEF_G, LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z, LD_LAPGRAD_X,
+                   LD_LAPGRAD_Y, LD_LAPGRAD_Z>;
+  static constexpr bool INV = true;
+  static constexpr int AX = kAxX;
+};
+template <>
+struct GroupOps<GRP_RHO> {
+  using L = OpList<LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
+  static constexpr bool INV = true;
+  static constexpr int AX = kAxX;
+};
+template <>
+struct GroupOps<GRP_PROJ> {
+  using L = OpList<LD_PRESSURE, LD_VEL0, LD_VEL1, LD_VEL2>;
+  static constexpr bool INV = true;
+  static constexpr int AX = kAxX;
+};
+template <>
+struct GroupOps<GRP_STRESS> {
+  using L = OpList<LD_STRESS, LD_STRESS, LD_STRESS>;
+  static constexpr bool INV = false;
+  static constexpr int AX = kAxZ;
+};
+template <>
+struct GroupOps<GRP_STB_PSI> {
+  using L = OpList<LD_SH, LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
+  static constexpr bool INV = true;
+  static constexpr int AX = kAxX;
+};
+template <>
+struct GroupOps<GRP_STB_PROJ> {
+  using L = OpList<LD_VEL0, LD_VEL1, LD_VEL2>;
+  static constexpr bool INV = true;
+  static constexpr int AX = kAxX;
+};
+template <>
+struct GroupOps<GRP_STB_STRESS> {
+  using L = OpList<LD_STRESS_STB, LD_STRESS_STB, LD_STRESS_STB>;
+  static constexpr bool INV = false;
+  static constexpr int AX = kAxZ;
+};
+
+template <int AX, int... LDs>
+constexpr int group_tables(OpList<LDs...>) {
+  return (kv_tables<AX, LDs, ST_PLAIN>() | ... | 0);
+}
+template <int... LDs>
+constexpr int group_count(OpList<LDs...>) {
+  return (int)sizeof...(LDs);
+}
+
+// Output O of a group: its pass on this tile, then the next output's.  The
+// passes index the same tile, so the compiler would keep every per-element
+// address of the first pass live for reuse in the next ones; an opaque copy
+// of the thread index per pass makes each pass derive its own, so each keeps
+// the register budget of a single pass.
+template <typename T, int N, int AX, bool INV, class SM, int O, int LD, int... REST>
+__device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& G, int tile,
+                                          SM& sm) {
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0>(G.geo, G.waves, G.phys, G.out[O].ld, G.out[O].st,
+                                            G.tw, nullptr, G.out[O].dst, tile, sm, O == 0, tid);
+  if constexpr (sizeof...(REST) > 0) {
+    __syncthreads();  // the next pass rewrites the tile in LDS
+    run_group<T, N, AX, INV, SM, O + 1>(OpList<REST...>{}, G, tile, sm);
+  }
+}
+
+template <typename T, int N, int GRP>
+using GroupSmem = Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
+                       smem_tables<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
+                                   group_tables<GroupOps<GRP>::AX>(typename GroupOps<GRP>::L{})>()>;
+
+// Waves per SIMD a group kernel keeps: z groups at least 3 (a register
+// budget of 168 VGPRs); x groups those of two workgroups where the LDS holds
+// two (as the single x passes do), else of one.
+template <typename T, int N, int GRP>
+constexpr int group_waves() {
+  constexpr int AX = GroupOps<GRP>::AX;
+  constexpr int w = (TileCfg<T, N, AX, 0>::BLOCK + 255) / 256;
+  if (AX == kAxZ) return w > 3 ? w : 3;
+  return GroupSmem<T, N, GRP>::bytes() <= 81920 ? 2 * w : w;
+}
+
+// The group runs the first pass of its transforms: the x pass of inverse
+// (x, y, z) transforms, the z pass of forward (z, y, x) ones.
+template <typename T, int N, int GRP>
+__global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(group_waves<T, N, GRP>())))
+    k_group(const GroupArgs G) {
+  using GO = GroupOps<GRP>;
+  constexpr int AX = GO::AX;
+  __shared__ GroupSmem<T, N, GRP> sm;
+  const int tile = tile_order<T, N, AX, 0>(G.geo, (int)blockIdx.x, (int)gridDim.x);
+  run_group<T, N, AX, GO::INV, decltype(sm), 0>(typename GO::L{}, G, tile, sm);
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
@@ -710,9 +819,10 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
     if (prefix[mid] <= b) lo = mid; else hi = mid;
   }
   __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
-  pass_tile<T, N, AX, INV, LD, ST, V>(descs[lo], b - prefix[lo], sm);
+  const PassArgs& A = descs[lo];
+  pass_tile<T, N, AX, INV, LD, ST, V>(A.geo, A.waves, A.phys, A.ld, A.st, A.tw, A.part, A.dst,
+                                      b - prefix[lo], sm);
 }
-
 
 template <typename T, int N, int AX, int V = 0>
 long tiles_of(const Geo& g) {
@@ -720,6 +830,35 @@ long tiles_of(const Geo& g) {
   if (AX == kAxZ) return ((long)g.nx * g.ny + C::TK - 1) / C::TK;
   const long tiles_k = (g.nz + C::TK - 1) / C::TK;
   return (AX == kAxY ? (long)g.nx : (long)g.ny) * tiles_k;
+}
+
+// group kernels of the groups that run on axis AX
+template <typename T, int N, int AX, int GG>
+hipError_t launch_group_g(const GroupArgs& G, long* nblocks, hipStream_t s) {
+  if constexpr (GroupOps<GG>::AX != AX) {
+    return hipErrorInvalidValue;
+  } else {
+    using C = TileCfg<T, N, AX, 0>;
+    const long nb = tiles_of<T, N, AX, 0>(G.geo);
+    if (nblocks) *nblocks = nb;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_group<T, N, GG>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, G);
+    return hipGetLastError();
+  }
+}
+
+template <typename T, int N, int AX>
+hipError_t launch_group_n(int group, const GroupArgs& G, long* nblocks, hipStream_t s) {
+  switch (group) {
+    case GRP_PSI: return launch_group_g<T, N, AX, GRP_PSI>(G, nblocks, s);
+    case GRP_RHO: return launch_group_g<T, N, AX, GRP_RHO>(G, nblocks, s);
+    case GRP_PROJ: return launch_group_g<T, N, AX, GRP_PROJ>(G, nblocks, s);
+    case GRP_STRESS: return launch_group_g<T, N, AX, GRP_STRESS>(G, nblocks, s);
+    case GRP_STB_PSI: return launch_group_g<T, N, AX, GRP_STB_PSI>(G, nblocks, s);
+    case GRP_STB_PROJ: return launch_group_g<T, N, AX, GRP_STB_PROJ>(G, nblocks, s);
+    case GRP_STB_STRESS: return launch_group_g<T, N, AX, GRP_STB_STRESS>(G, nblocks, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // Single pass (B == nullptr) or batched pass (B: device descriptors).

@@ -29,4 +29,25 @@ hipError_t QSM_CAT(launch_pass, QSM_T, QSM_AX)(int n, bool inv, int ld, int st, 
   }
 }
 
+#if QSM_AX == 0 || QSM_AX == 2
+template <int N, bool STEP>
+static hipError_t launch_group_if(int group, const GroupArgs& G, long* nb, hipStream_t s) {
+  if constexpr (STEP) return launch_group_n<QSM_T, N, QSM_AX>(group, G, nb, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t QSM_CAT(launch_group, QSM_T, QSM_AX)(int n, int group, const GroupArgs& G, long* nb,
+                                                hipStream_t s) {
+  switch (n) {
+#define QSM_CASE(NN) \
+  case NN:           \
+    return launch_group_if<NN, QSM_IS_STEP_N(NN)>(group, G, nb, s);
+    QSM_FOR_EACH_N(QSM_CASE)
+#undef QSM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+#endif
+
 }  // namespace qsm
