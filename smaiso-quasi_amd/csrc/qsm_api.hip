@@ -80,9 +80,9 @@ const void* twiddles(int n, int fp32) {
   auto put = [&](double re, double im) { h.push_back(re); h.push_back(im); };
   double re, im;
   for (int v = 0; v < m; ++v) { twq(8 * v, re, im); put(re, im); }  // e^{-2 pi i v/M}
-  for (int k = 0; k < m; ++k) {                                      // conj twq(k), conj twq(4k)
+  for (int k = 0; k < m; ++k) {  // conj twq(k) = e^{i pi k/2N}, conj twq(5k+N) = i e^{2 pi i k/N} e^{i pi k/2N}
     twq(k, re, im); put(re, -im);
-    twq(4 * k, re, im); put(re, -im);
+    twq(5 * k + n, re, im); put(re, -im);
   }
   for (int k = 0; k <= m / 2; ++k) {                                 // twq(4k), twq(k), twq(M-k)
     twq(4 * k, re, im); put(re, im);

@@ -154,7 +154,8 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 // (twq(u) = e^{-2 pi i u/(4N)} below):
 //   [0, M)               stage twiddles    e^{-2 pi i v/M} = twq(8v)
 //   [M, 3M)              DCT-III pre-processing, per m < M:
-//                        conj twq(m), conj twq(4m)
+//                        conj twq(m) = e^{i pi m/2N},
+//                        conj twq(5m+N) = i e^{2 pi i m/N} e^{i pi m/2N}
 //   [3M, 3M + 3(M/2+1))  DCT-II post-processing, per k <= M/2:
 //                        twq(4k), twq(k), twq(M-k)
 // A pass stages the stage part and its direction's part in LDS; every

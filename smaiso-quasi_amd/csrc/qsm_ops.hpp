@@ -107,6 +107,22 @@ __device__ __forceinline__ void st(void* base, Ix idx, T v) {
   *(T*)((char*)base + off) = v;
 }
 
+// Split addressing: element thr + uni of a field, with thr a per-thread
+// offset (>= 0) and uni an offset uniform across the wavefront (may be
+// negative; thr + uni >= 0).  The uniform part is added to the base pointer
+// with scalar instructions and the per-thread part is the 32-bit vector
+// offset, so the accesses of a walk that share thr share one offset register.
+template <typename T>
+__device__ __forceinline__ T ld_at(const void* base, Ix thr, int uni) {
+  const char* b = (const char*)base + (long)uni * (long)sizeof(T);
+  return *(const T*)(b + thr * (Ix)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(void* base, Ix thr, int uni, T v) {
+  char* b = (char*)base + (long)uni * (long)sizeof(T);
+  *(T*)(b + thr * (Ix)sizeof(T)) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ double rd(const void* p, Ix idx) {
   return (double)ld<T>(p, idx);
@@ -245,9 +261,41 @@ __device__ __forceinline__ Raw4<T> fetch(const LoadOp& L, const Geo& G, int i, i
   return r;
 }
 
+// Mixed precision (T = float): the projection reads FP32 fields and writes
+// an FP32 field, so it evaluates in FP32 with the hardware reciprocal
+// (1 ulp); K^2 is a sum of squares (no cancellation).
+template <int LD>
+__device__ __forceinline__ float finish32(const Raw4<float>& r, const LoadOp& L, const Phys& P,
+                                          const Kv& K, const Geo& G, int i, int j, int k) {
+  const Shifts h = shifts<LD>(L, G, i, j, k);
+  const float a = h.fa ? r.a : 0.0f, b = h.fb ? r.b : 0.0f, c = h.fc ? r.c : 0.0f,
+              dd = h.fd ? r.d : 0.0f;
+  const float nu = (float)P.nu, scale = (float)P.scale;
+  if constexpr (LD == LD_PRESSURE) {
+    const float qx = (float)K.qx, qy = (float)K.qy, qz = (float)K.qz;
+    const float rm2 = __builtin_amdgcn_rcpf(qx * qx + qy * qy + qz * qz);
+    const float pv = -scale * (qx * a + qy * b + qz * c) * rm2;
+    return (i + j + k == 0) ? 0.0f : pv;
+  } else {
+    // own component: wavenumber s_c; the others q
+    constexpr int cc = LD == LD_VEL0 ? 0 : (LD == LD_VEL1 ? 1 : 2);
+    const float kx = (float)(cc == 0 ? K.sx : K.qx), ky = (float)(cc == 1 ? K.sy : K.qy),
+                kz = (float)(cc == 2 ? K.sz : K.qz);
+    const float rm2 = __builtin_amdgcn_rcpf(kx * kx + ky * ky + kz * kz);
+    const float ks = cc == 0 ? kx : (cc == 1 ? ky : kz);
+    // read a is the own component; b, c the others in axis order
+    const float dot = cc == 0 ? kx * a + ky * b + kz * c
+                              : (cc == 1 ? kx * b + ky * a + kz * c : kx * b + ky * c + kz * a);
+    const float vv = (scale / nu) * rm2 * (a - ks * rm2 * dot + nu * ks * dd);
+    const bool in = cc == 0 ? i < G.nx - 1 : ((j < G.ny - 1) & (k < G.nz - 1));
+    return in ? vv : 0.0f;
+  }
+}
+
 template <typename T, int LD>
 __device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phys& P, const Kv& K,
                                     const Geo& G, int i, int j, int k) {
+  if constexpr (sizeof(T) == sizeof(float)) return finish32<LD>(r, L, P, K, G, i, j, k);
   const Shifts h = shifts<LD>(L, G, i, j, k);
   const double a = h.fa ? (double)r.a : 0.0, b = h.fb ? (double)r.b : 0.0,
                c = h.fc ? (double)r.c : 0.0, dd = h.fd ? (double)r.d : 0.0;
@@ -285,10 +333,39 @@ __device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phy
 template <typename T, int LD>
 __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv& K,
                                        const Geo& G, int i, int j, int k, Ix idx) {
+  constexpr bool MIX = sizeof(T) == sizeof(float);  // FP32 fields: mixed precision
   if constexpr (LD == LD_PLAIN) {
     return ld<T>(L.a[0], idx);
   } else if constexpr (LD == LD_ZERO) {
     return (T)0;
+  } else if constexpr (MIX && (LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH)) {
+    // mixed precision: the coefficient in FP64 (q0^2 - K^2 cancels near the
+    // ring), rounded once, times the FP32 input
+    const double mq2 = sq(K.qx) + (sq(K.qy) + sq(K.qz));
+    const double d = P.q02 - mq2;
+    const T x = ld<T>(L.a[0], idx);
+    if constexpr (LD == LD_SH) return (T)(P.alpha * sq(d) - P.eps) * x;
+    const T fh = (T)(P.alpha * d) * x;
+    return LD == LD_COEF_F ? fh : (T)(-mq2) * fh;
+  } else if constexpr (MIX && LD >= LD_GRAD_X && LD <= LD_LAPGRAD_Z) {
+    // mixed precision: one FP64 factor per element, rounded once, times the
+    // FP32 input (grad: -s_c; lap grad: K_c^2 s_c with K_c the wavenumbers
+    // of the shifted component)
+    constexpr int c = (LD - LD_GRAD_X) % 3;
+    const double st = c == 0 ? K.sx : (c == 1 ? K.sy : K.sz);
+    double f = -st;
+    if constexpr (LD >= LD_LAPGRAD_X) {
+      const double ax = c == 0 ? K.sx : K.qx, ay = c == 1 ? K.sy : K.qy, az = c == 2 ? K.sz : K.qz;
+      // these operators run on the x pass: the y, z terms are per thread
+      const double m = sq(ax) + (sq(ay) + sq(az));
+      f = m * st;
+    }
+    const int m = c == 0 ? i : (c == 1 ? j : k);
+    const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
+    const int o = c == 0 ? G.ny * G.nz : (c == 1 ? G.nz : 1);
+    const bool ok = m < nc - 1;
+    const T x = ld<T>(L.a[0], ok ? idx + (Ix)o : idx);
+    return ok ? (T)f * x : (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
     const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
     const double fh = P.alpha * (P.q02 - mq2) * (double)ld<T>(L.a[0], idx);

@@ -304,6 +304,17 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
 
   TileGeo<AX, N, TK, BLOCK> tg;
   tg.init(G, tile, tid);
+  // Split addressing of the walk for plain loads and stores: element s at
+  // the per-thread tg.wbase plus the uniform walk_uni(s).  In z tiles of
+  // N % BLOCK == 0, element s lies in row row0 + s / R of the tile (uniform);
+  // rows past the field read the tile's first row instead.
+  constexpr bool SPLIT = AX != kAxZ || N % BLOCK == 0;
+  constexpr int RZ = (AX == kAxZ && N % BLOCK == 0) ? N / BLOCK : 1;
+  auto row_ok = [&](int s) { return AX != kAxZ || tg.row0 + s / RZ < tg.nrows; };
+  auto walk_uni = [&](int s) -> int {
+    if constexpr (AX != kAxZ) return s * (int)tg.wstep;
+    else return row_ok(s) ? s * BLOCK : (s % RZ) * BLOCK;
+  };
 
   if (stage) {
     const cx<T>* __restrict__ tw = (const cx<T>*)twp;
@@ -401,16 +412,23 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
+          if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
+          else xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
         }
+        // values of rows or columns outside the field (read from a valid
+        // address) go through the transform of their own pencil only and
+        // are never stored; only the DC sum must leave them out
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           int q, n, i, j, k;
           Ix idx;
           bool ok, oddn;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          T x = ok ? xs[c] : (T)0;
-          if constexpr (DSUM) dsum += (double)x;
+          T x = xs[c];
+          if constexpr (DSUM) {
+            if (!ok) x = (T)0;
+            dsum += (double)x;
+          }
           const int lp = wp.pos(tid, s0 + c, oddn);
           if (dst & oddn) x = -x;
           rbuf[lp] = x;
@@ -457,10 +475,14 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
     // N-1-o swaps the two families)
     PencilPos<TKP> pp;
     pp.init(p, t);
+    // split addressing of the direct stores: thread bases at samples t and
+    // M - t (inside the field), uniform offsets along the axis
     Ix gu = 0, gd = 0;
+    int ax = 0;
     if constexpr (!STAGED) {
       gu = tg.cbase + (Ix)t * tg.axs;
-      gd = tg.cbase - (Ix)t * tg.axs;
+      gd = tg.cbase + (Ix)(M - t) * tg.axs;
+      ax = (int)tg.axs;
     }
     auto put_k = [&](bool up, int K, T val) {
       if constexpr (STAGED) {
@@ -468,9 +490,16 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
         const int pb = up ? pp.dn(N - 1 - K) : pp.up(N - 1 - K);
         rbuf[dst ? pb : pa] = val;
       } else {
-        const Ix ia = (up ? gu : gd) + (Ix)K * tg.axs;
-        const Ix ib = (up ? gd : gu) + (Ix)(N - 1 - K) * tg.axs;
-        if (tg.colok) store_eval<T, ST>(Sop, dst ? ib : ia, val);
+        // sample t + K (up) or K - t (down); DST: N-1 minus that
+        if (tg.colok) {
+          if (dst) {
+            if (up) st_at<T>(Sop.d[0], gd, (N - 1 - K - M) * ax, val);
+            else st_at<T>(Sop.d[0], gu, (N - 1 - K) * ax, val);
+          } else {
+            if (up) st_at<T>(Sop.d[0], gu, K * ax, val);
+            else st_at<T>(Sop.d[0], gd, (K - M) * ax, val);
+          }
+        }
       }
     };
     if constexpr (STAGED) __syncthreads();
@@ -507,15 +536,21 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
         }
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          int q, n, i, j, k;
-          Ix idx;
-          bool ok;
-          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          if (!ok) continue;
           bool oddn;
-          const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
-          if constexpr (store_reads<ST>()) store_apply<T, ST>(Sop, Ph, kv_at(i, j, k, q), idx, y, in[c], dcv);
-          else store_eval<T, ST>(Sop, idx, y);
+          if constexpr (!store_reads<ST>() && SPLIT && AX == kAxZ) {
+            if (!row_ok(s0 + c)) continue;
+            const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
+            st_at<T>(Sop.d[0], tg.wbase, (s0 + c) * BLOCK, y);
+          } else {
+            int q, n, i, j, k;
+            Ix idx;
+            bool ok;
+            tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+            if (!ok) continue;
+            const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
+            if constexpr (store_reads<ST>()) store_apply<T, ST>(Sop, Ph, kv_at(i, j, k, q), idx, y, in[c], dcv);
+            else store_eval<T, ST>(Sop, idx, y);
+          }
         }
       }
     }
@@ -556,27 +591,30 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
+            if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
+            else xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
           }
         }
+        // values outside the field stay in their own pencil and are never
+        // stored; dst (uniform) picks the reversed positions
+        bool oddn;
+        if (dst) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          int q, n, i, j, k;
-          Ix idx;
-          bool ok;
-          tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          const T x = ok ? xs[c] : (T)0;
-          bool oddn;
-          const int pn = wn.pos(tid, s0 + c, oddn), pr = wr.pos(tid, s0 + c, oddn);
-          rbuf[dst ? pr : pn] = x;
+          for (int c = 0; c < CH; ++c) rbuf[wr.pos(tid, s0 + c, oddn)] = xs[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < CH; ++c) rbuf[wn.pos(tid, s0 + c, oddn)] = xs[c];
         }
       }
     }
     __syncthreads();
     // ---- DCT-III pre-processing straight into the stage-1 inputs:
     // Z_m = (V_m + V_{m+M}) + i e^{2 pi i m/N} (V_m - V_{m+M}),
-    // V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m})
-    const cx<T> r8 = {(T)0.70710678118654752440, (T)0.70710678118654752440};
+    // V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m}).
+    // With a = X_m - i X_{N-m}, c = e^{i pi/4}(X_{M+m} - i X_{M-m}):
+    // Z_m = A (a + c) + B (a - c),  A = e^{i pi m/2N},  B = i e^{2 pi i m/N} A = e^{i pi (5m+N)/2N}
+    // (tp[2m], tp[2m+1]).
+    const T h = (T)0.70710678118654752440;
     // samples m, N-m, M+m, M-m of item m = t + C from per-thread bases
     PencilPos<TKP> pp;
     pp.init(p, t);
@@ -590,12 +628,13 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
         T XNm;  // X_N = 0
         if (C == 0) XNm = t == 0 ? (T)0 : rbuf[t == 0 ? pp.up(0) : pp.dn(N)];
         else XNm = rbuf[pp.dn(N - C)];
-        const cx<T> w1 = tp[2 * m];  // e^{i pi m/2N}
-        const cx<T> vk = cmul(w1, cx<T>{Xm, -XNm});
-        const cx<T> vm = cmul(cmul(w1, r8), cx<T>{XMp, -XMm});
-        const cx<T> Ev = vk + vm;
-        const cx<T> Ov = cmul(tp[2 * m + 1], vk - vm);  // e^{2 pi i m/N}
-        v[b * R1 + r] = Ev + mul_si<1>(Ov);
+        // c = h (d1 + i d2)
+        const T d1 = XMp + XMm, d2 = XMp - XMm;
+        const cx<T> P = {fma(h, d1, Xm), fma(h, d2, -XNm)};
+        const cx<T> Q = {fma(-h, d1, Xm), fma(-h, d2, -XNm)};
+        const cx<T> A = tp[2 * m], B = tp[2 * m + 1];
+        v[b * R1 + r] = {fma(A.x, P.x, fma(-A.y, P.y, fma(B.x, Q.x, -B.y * Q.y))),
+                         fma(A.x, P.y, fma(A.y, P.x, fma(B.x, Q.y, B.y * Q.x)))};
       }
     }
     if constexpr (AX == kAxZ) {
@@ -605,14 +644,22 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       wp.init(tid);
 #pragma unroll
       for (int s = 0; s < PER; ++s) {
-        int q, n, i, j, k;
-        Ix idx;
-        bool ok, oddn;
-        tg.walk(tid, s, q, n, i, j, k, idx, ok);
-        if (!ok) continue;
-        T y = rbuf[wp.pos(tid, s, oddn)];
-        if (dst & oddn) y = -y;
-        store_eval<T, ST>(Sop, idx, y);
+        bool oddn;
+        if constexpr (SPLIT) {
+          if (!row_ok(s)) continue;
+          T y = rbuf[wp.pos(tid, s, oddn)];
+          if (dst & oddn) y = -y;
+          st_at<T>(Sop.d[0], tg.wbase, s * BLOCK, y);
+        } else {
+          int q, n, i, j, k;
+          Ix idx;
+          bool ok;
+          tg.walk(tid, s, q, n, i, j, k, idx, ok);
+          if (!ok) continue;
+          T y = rbuf[wp.pos(tid, s, oddn)];
+          if (dst & oddn) y = -y;
+          store_eval<T, ST>(Sop, idx, y);
+        }
       }
     } else {
       run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, v, buf, p, t, ts);
@@ -622,6 +669,10 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       constexpr int RL = Last<typename PL::R>::value, NBL = E / RL;
       if (tg.colok) {
         const Ix g4u = tg.cbase + (Ix)(4 * t) * tg.axs, g4d = tg.cbase - (Ix)(4 * t) * tg.axs;
+        // split addressing: samples 4t (+ 2 u0) and N-1-4t (+ N - 2 u0), both
+        // thread bases inside the field (4t < N)
+        const Ix tu = g4u, td = tg.cbase + (Ix)(N - 1 - 4 * t) * tg.axs;
+        const int ax = (int)tg.axs;
 #pragma unroll
         for (int b = 0; b < NBL; ++b) {
 #pragma unroll
@@ -632,11 +683,16 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             for (int c = 0; c < 2; ++c) {
               const int u0 = 2 * C + c;
               const T y = c ? z.y : z.x;
-              const Ix iu = g4u + (Ix)(2 * u0) * tg.axs;
-              const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
-              // constant when the thread range t < TP stays on one side of M
-              const bool low = (2 * (TP - 1) + u0 < M) ? true : (u0 >= M ? false : 2 * t + u0 < M);
-              store_eval<T, ST>(Sop, low ? iu : idn, (!low && dst) ? -y : y);
+              if (2 * (TP - 1) + u0 < M) {  // every thread: n = 4t + 2 u0
+                st_at<T>(Sop.d[0], tu, 2 * u0 * ax, y);
+              } else if (u0 >= M) {  // every thread: n = 2N-1-2u0-4t
+                st_at<T>(Sop.d[0], td, (N - 2 * u0) * ax, dst ? -y : y);
+              } else {  // the thread range straddles M
+                const Ix iu = g4u + (Ix)(2 * u0) * tg.axs;
+                const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
+                const bool low = 2 * t + u0 < M;
+                store_eval<T, ST>(Sop, low ? iu : idn, (!low && dst) ? -y : y);
+              }
             }
           }
         }
@@ -782,15 +838,18 @@ using GroupSmem = Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
                        smem_tables<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
                                    group_tables<GroupOps<GRP>::AX>(typename GroupOps<GRP>::L{})>()>;
 
-// Waves per SIMD a group kernel keeps: z groups at least 3 (a register
-// budget of 168 VGPRs); x groups those of two workgroups where the LDS holds
-// two (as the single x passes do), else of one.
+// Waves per SIMD a group kernel asks the register allocator for: z groups
+// at least 3 (a budget of 168 VGPRs), x groups those of two workgroups (as
+// the single x passes run), both capped at what the LDS (160 KB per CU)
+// allows and at least one workgroup's.
 template <typename T, int N, int GRP>
 constexpr int group_waves() {
   constexpr int AX = GroupOps<GRP>::AX;
   constexpr int w = (TileCfg<T, N, AX, 0>::BLOCK + 255) / 256;
-  if (AX == kAxZ) return w > 3 ? w : 3;
-  return GroupSmem<T, N, GRP>::bytes() <= 81920 ? 2 * w : w;
+  constexpr int lds = (int)(163840 / sizeof(GroupSmem<T, N, GRP>)) * w;
+  constexpr int want = AX == kAxZ ? (w > 3 ? w : 3) : 2 * w;
+  constexpr int r = want < lds ? want : lds;
+  return r < w ? w : (r > 8 ? 8 : r);
 }
 
 // The group runs the first pass of its transforms: the x pass of inverse
