@@ -436,11 +436,11 @@ int fwd3(qsm_handle_s* h, int dst_axis, const LoadOp& L, void* tmp, const StoreO
   return run_pass(h, 0, false, dst_axis == 0, plain_load(tmp), S, false, nullptr);
 }
 
-// ---- group transforms: several 3D transforms whose first pass reads the
-// same fields run that pass as one kernel (launch_group), then their other
-// passes one by one.  Inverse groups run x first (x, y, z, as inv3), forward
-// groups z first (z, y, x, as fwd3).  Used by single handles; an ensemble
-// records its passes one transform at a time and keeps the per-transform path.
+// ---- group transforms: several inverse 3D transforms whose first (x) pass
+// reads the same fields run that pass as one kernel (launch_group), then
+// their y and z passes one by one.  Used by single handles, as are the
+// multi-output forward transforms below; an ensemble records its passes one
+// transform at a time and keeps the per-transform path.
 struct GroupItem {
   LoadOp L;
   void* out;     // output field; the group pass writes it, the others run in place
@@ -488,15 +488,40 @@ int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int r
   return rc;
 }
 
-// forward transforms z, y, x with plain stores (the z pass as one group kernel)
-int fwd3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
-  int rc = run_group_first(h, group, it, count, reads);
-  for (int o = 0; o < count && !rc; ++o) {
-    rc = run_pass(h, 1, false, it[o].dst_axis == 1, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
-    if (!rc)
-      rc = run_pass(h, 0, false, it[o].dst_axis == 0, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
+// Forward transforms z, y, x of the three components of a multi-output
+// operator (L: LD_STRESS3 / LD_STRESS_STB3, component c with DST along c):
+// the z pass of all three as one kernel reading the shared inputs once
+// (launch_multi), then y and x in place per component.  `reads`: distinct
+// fields the operator reads (algorithmic bytes).  Returns 1 (no error set)
+// where the pencil length has no multi-output kernel.
+int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const out[3], int reads) {
+  if (!multi_supported(h->fp32, h->p.nz)) return 1;
+  if (!h->dry) {
+    if (h->rec) return fail(QSM_ERR_STATE, "multi-output passes cannot be recorded");
+    MultiArgs A{};
+    A.geo = h->geo;
+    A.phys = h->phys;
+    A.ld = L;
+    A.tw = h->tw[2];
+    A.dmask = 1 << 2;  // component z: DST along z
+    for (int c = 0; c < 3; ++c) A.out[c] = out[c];
+    Timer& T = h->timer;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+    hipError_t e = launch_multi(h->fp32, h->p.nz, A, nullptr, h->stream);
+    if (e != hipSuccess) return fail(QSM_ERR_HIP, "multi-output launch: %s", hipGetErrorString(e));
+    if (T.on) {
+      hipEventRecord(eb, h->stream);
+      T.recs.push_back({ea, eb, timer_key(2, false, L.code, ST_PLAIN, 0),
+                        (double)(reads + 3) * (double)h->n * (double)h->esz});
+    }
   }
-  return rc;
+  for (int c = 0; c < 3; ++c) {
+    int rc = run_pass(h, 1, false, c == 1, plain_load(out[c]), plain_store(out[c]), false, nullptr);
+    if (!rc) rc = run_pass(h, 0, false, c == 0, plain_load(out[c]), plain_store(out[c]), false, nullptr);
+    if (rc) return rc;
+  }
+  return QSM_OK;
 }
 
 void* F(qsm_handle_s* h, int f) { return h->f[f]; }
@@ -581,16 +606,15 @@ int quasi_step(qsm_handle_s* h) {
   }
 
   // S5: stress (quasi.cpp:1298-1330)
+  rc = 1;
   if (grp) {
-    GroupItem it[3];
-    for (int c = 0; c < 3; ++c) {
-      it[c] = {LoadOp{}, F(h, F_SX + c), c};
-      it[c].L.code = LD_STRESS; it[c].L.axis = c;
-      it[c].L.a[0] = rho; it[c].L.a[1] = F(h, F_G); it[c].L.a[2] = F(h, F_F);
-      it[c].L.a[3] = F(h, F_GX + c); it[c].L.a[4] = F(h, F_LX + c);
-    }
-    if ((rc = fwd3_group(h, GRP_STRESS, it, 3, 9))) return rc;
-  } else {
+    L = LoadOp{}; L.code = LD_STRESS3;
+    L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F);
+    for (int c = 0; c < 3; ++c) { L.a[3 + c] = F(h, F_GX + c); L.a[6 + c] = F(h, F_LX + c); }
+    void* const out[3] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ)};
+    if ((rc = fwd3_multi(h, L, out, 9)) < 0) return rc;
+  }
+  if (rc) {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
       L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F); L.a[3] = F(h, F_GX + c); L.a[4] = F(h, F_LX + c);
@@ -677,12 +701,19 @@ int stb_step(qsm_handle_s* h) {
     }
     for (auto& x : it) x.L.a[0] = psih;
     if ((rc = inv3_group(h, GRP_STB_PSI, it, 4, 1))) return rc;
-    for (int c = 0; c < 3; ++c) {
-      it[c] = {LoadOp{}, F(h, F_SX + c), c};
-      it[c].L.code = LD_STRESS_STB; it[c].L.axis = c;
-      it[c].L.a[0] = F(h, F_F); it[c].L.a[1] = psi; it[c].L.a[2] = F(h, F_GX + c);
+    L = LoadOp{}; L.code = LD_STRESS_STB3;
+    L.a[0] = F(h, F_F); L.a[1] = psi;
+    for (int c = 0; c < 3; ++c) L.a[2 + c] = F(h, F_GX + c);
+    void* const out[3] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ)};
+    if ((rc = fwd3_multi(h, L, out, 5)) == 1) {
+      for (int c = 0; c < 3; ++c) {
+        L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
+        L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);
+        if ((rc = fwd3(h, c, L, F(h, F_SX + c), plain_store(F(h, F_SX + c)), -1))) return rc;
+      }
+    } else if (rc) {
+      return rc;
     }
-    if ((rc = fwd3_group(h, GRP_STB_STRESS, it, 3, 5))) return rc;
     for (int c = 0; c < 3; ++c) {
       it[c] = {LoadOp{}, F(h, F_VX + c), c};
       it[c].L.code = LD_VEL0 + c; it[c].L.n0 = n0;

@@ -54,19 +54,16 @@ bool step_supported(int n);
 int pass_tile_k(int prec_fp32, int n, int axis, int variant = 0);
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 
-// Group passes.  One kernel runs the first pass of several transforms that
-// read the same fields: each workgroup runs them one after another on its
-// tile, so the shared inputs come from HBM once and from L2 after that.
-// Inverse groups run the x pass of x, y, z inverse transforms, forward groups
-// the z pass of z, y, x forward transforms (group_axis).
+// Group passes.  One kernel runs the first (x) pass of several inverse
+// transforms that read the same fields: each workgroup runs them one after
+// another on its tile, so later outputs re-read the shared inputs from the
+// caches.
 enum GroupCode : int {
-  GRP_PSI = 0,     // inverse: F, G, grad psi (x, y, z), lap grad psi (x, y, z) from psihat
-  GRP_RHO,         // inverse: grad rho (x, y, z) from rhohat
-  GRP_PROJ,        // inverse: p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
-  GRP_STRESS,      // forward: S_x, S_y, S_z
-  GRP_STB_PSI,     // inverse: sh, grad psi (x, y, z)
-  GRP_STB_PROJ,    // inverse: v_x, v_y, v_z
-  GRP_STB_STRESS,  // forward: S_x, S_y, S_z of the stb step
+  GRP_PSI = 0,     // F, G, grad psi (x, y, z), lap grad psi (x, y, z) from psihat
+  GRP_RHO,         // grad rho (x, y, z) from rhohat
+  GRP_PROJ,        // p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
+  GRP_STB_PSI,     // sh, grad psi (x, y, z)
+  GRP_STB_PROJ,    // v_x, v_y, v_z
   GRP_COUNT
 };
 constexpr int kMaxGroup = 8;
@@ -83,10 +80,26 @@ struct GroupArgs {
   const void* tw;  // twiddle table of the group's axis
   GroupOut out[kMaxGroup];
 };
+// Multi-output forward z pass (the three stress components from one read
+// of their shared inputs): ld is a multi-output operator (LD_STRESS3,
+// LD_STRESS_STB3), out[c] receives component c, dmask bit c selects DST
+// along z for component c.
+struct MultiArgs {
+  Geo geo;
+  Phys phys;
+  LoadOp ld;
+  void* out[3];
+  const void* tw;  // z-axis twiddle table
+  int dmask;
+  int pad_;
+};
+hipError_t launch_multi(int prec_fp32, int n, const MultiArgs& A, long* nblocks, hipStream_t s);
+bool multi_supported(int prec_fp32, int n);
+
 // operators of a group in output order, their count, and the direction
 int group_ops(int group, int* codes);
 bool group_inverse(int group);
-int group_axis(int group);  // 0 (x) or 2 (z)
+int group_axis(int group);  // 0 (x)
 hipError_t launch_group(int prec_fp32, int n, int group, const GroupArgs& G, long* nblocks,
                         hipStream_t s);
 

@@ -61,6 +61,9 @@ enum LoadCode : int {
   LD_STRESS_STB,    // rho mu g_c, mu = sh - beta psi^3 + gamma psi^5     quasi-stb.cpp:1295-1307
   LD_NONLIN_STB,    // beta psi^3 - gamma psi^5 - v.g (hydro = 1)         quasi-stb.cpp:1903-1914
   LD_NONLIN_STB0,   // beta psi^3 - gamma psi^5 (hydro = 0)               quasi-stb.cpp:1915-1919
+  // multi-output operators (three values per element, one per component c)
+  LD_STRESS3,       // LD_STRESS for c = x, y, z; a: rho, G, F, g_x,y,z, l_x,y,z
+  LD_STRESS_STB3,   // LD_STRESS_STB for c = x, y, z; a: sh, psi, g_x,y,z
 };
 
 enum StoreCode : int {
@@ -121,6 +124,13 @@ template <typename T>
 __device__ __forceinline__ void st_at(void* base, Ix thr, int uni, T v) {
   char* b = (char*)base + (long)uni * (long)sizeof(T);
   *(T*)(b + thr * (Ix)sizeof(T)) = v;
+}
+
+// non-temporal form of st_at (streaming store, low L2 retention)
+template <typename T>
+__device__ __forceinline__ void st_at_nt(void* base, Ix thr, int uni, T v) {
+  char* b = (char*)base + (long)uni * (long)sizeof(T);
+  __builtin_nontemporal_store(v, (T*)(b + thr * (Ix)sizeof(T)));
 }
 
 template <typename T>
@@ -440,6 +450,31 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
   } else {
     static_assert(LD == LD_PLAIN, "unknown load operator");
     return (T)0;
+  }
+}
+
+// Multi-output operators: the three components of the stress at one
+// element, reading the shared inputs once.  Each value is the expression of
+// the single-output operator (LD_STRESS, LD_STRESS_STB) term for term.
+template <int LD>
+constexpr int multi_count() {
+  return (LD == LD_STRESS3 || LD == LD_STRESS_STB3) ? 3 : 1;
+}
+template <typename T, int LD>
+__device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx, T (&out)[3]) {
+  if constexpr (LD == LD_STRESS3) {
+    const T r = ld<T>(L.a[0], idx);
+    const T gg = r * ld<T>(L.a[1], idx);
+    const T rf = r * ld<T>(L.a[2], idx);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = gg * ld<T>(L.a[3 + c], idx) - rf * ld<T>(L.a[6 + c], idx);
+  } else {
+    static_assert(LD == LD_STRESS_STB3, "not a multi-output operator");
+    const T ps = ld<T>(L.a[1], idx);
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    const T mu = ld<T>(L.a[0], idx) - (T)P.beta * ps3 + (T)P.gamma * ps5;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = mu * ld<T>(L.a[2 + c], idx);
   }
 }
 

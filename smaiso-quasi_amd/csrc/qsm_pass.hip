@@ -19,17 +19,13 @@ QSM_DECL(double, 2)
 #undef QSM_DECL
 
 hipError_t launch_group_float_0(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
-hipError_t launch_group_float_2(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
 hipError_t launch_group_double_0(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
-hipError_t launch_group_double_2(int n, int group, const GroupArgs& G, long* nb, hipStream_t s);
 
 hipError_t launch_group(int prec_fp32, int n, int group, const GroupArgs& G, long* nblocks,
                         hipStream_t s) {
-  if (group_axis(group) == kAxX)
-    return prec_fp32 ? launch_group_float_0(n, group, G, nblocks, s)
-                     : launch_group_double_0(n, group, G, nblocks, s);
-  return prec_fp32 ? launch_group_float_2(n, group, G, nblocks, s)
-                   : launch_group_double_2(n, group, G, nblocks, s);
+  if (group_axis(group) != kAxX) return hipErrorInvalidValue;
+  return prec_fp32 ? launch_group_float_0(n, group, G, nblocks, s)
+                   : launch_group_double_0(n, group, G, nblocks, s);
 }
 
 template <int... LDs>
@@ -44,10 +40,8 @@ int group_ops(int group, int* codes) {
     case GRP_PSI: return list_codes(GroupOps<GRP_PSI>::L{}, codes);
     case GRP_RHO: return list_codes(GroupOps<GRP_RHO>::L{}, codes);
     case GRP_PROJ: return list_codes(GroupOps<GRP_PROJ>::L{}, codes);
-    case GRP_STRESS: return list_codes(GroupOps<GRP_STRESS>::L{}, codes);
     case GRP_STB_PSI: return list_codes(GroupOps<GRP_STB_PSI>::L{}, codes);
     case GRP_STB_PROJ: return list_codes(GroupOps<GRP_STB_PROJ>::L{}, codes);
-    case GRP_STB_STRESS: return list_codes(GroupOps<GRP_STB_STRESS>::L{}, codes);
     default: return 0;
   }
 }
@@ -55,18 +49,29 @@ int group_ops(int group, int* codes) {
 bool group_inverse(int group) {
   switch (group) {
 #define QSM_G(GG) case GG: return GroupOps<GG>::INV;
-    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STRESS) QSM_G(GRP_STB_PSI)
-    QSM_G(GRP_STB_PROJ) QSM_G(GRP_STB_STRESS)
+    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_STB_PROJ)
 #undef QSM_G
     default: return false;
   }
 }
 
+hipError_t launch_multi_float_n(int n, const MultiArgs& A, long* nb, hipStream_t s);
+hipError_t launch_multi_double_n(int n, const MultiArgs& A, long* nb, hipStream_t s);
+bool multi_ok_float_n(int n);
+bool multi_ok_double_n(int n);
+
+hipError_t launch_multi(int prec_fp32, int n, const MultiArgs& A, long* nblocks, hipStream_t s) {
+  return prec_fp32 ? launch_multi_float_n(n, A, nblocks, s) : launch_multi_double_n(n, A, nblocks, s);
+}
+
+bool multi_supported(int prec_fp32, int n) {
+  return prec_fp32 ? multi_ok_float_n(n) : multi_ok_double_n(n);
+}
+
 int group_axis(int group) {
   switch (group) {
 #define QSM_G(GG) case GG: return GroupOps<GG>::AX;
-    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STRESS) QSM_G(GRP_STB_PSI)
-    QSM_G(GRP_STB_PROJ) QSM_G(GRP_STB_STRESS)
+    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_STB_PROJ)
 #undef QSM_G
     default: return -1;
   }

@@ -269,7 +269,7 @@ using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<A
 // One workgroup's work: tile `tile` of the pass described by A, in the LDS
 // sm.  stage = false skips the staging of twiddles and tables a previous
 // pass of the same kernel (same axis and direction) left in sm.
-template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM>
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM, bool NTS = false>
 __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Phys& Ph,
                                           const LoadOp& Lop, const StoreOp& Sop,
                                           const void* twp, double* part, const int dstf,
@@ -684,9 +684,11 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
               const int u0 = 2 * C + c;
               const T y = c ? z.y : z.x;
               if (2 * (TP - 1) + u0 < M) {  // every thread: n = 4t + 2 u0
-                st_at<T>(Sop.d[0], tu, 2 * u0 * ax, y);
+                if constexpr (NTS) st_at_nt<T>(Sop.d[0], tu, 2 * u0 * ax, y);
+                else st_at<T>(Sop.d[0], tu, 2 * u0 * ax, y);
               } else if (u0 >= M) {  // every thread: n = 2N-1-2u0-4t
-                st_at<T>(Sop.d[0], td, (N - 2 * u0) * ax, dst ? -y : y);
+                if constexpr (NTS) st_at_nt<T>(Sop.d[0], td, (N - 2 * u0) * ax, dst ? -y : y);
+                else st_at<T>(Sop.d[0], td, (N - 2 * u0) * ax, dst ? -y : y);
               } else {  // the thread range straddles M
                 const Ix iu = g4u + (Ix)(2 * u0) * tg.axs;
                 const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
@@ -713,6 +715,170 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       for (int w = 0; w < (BLOCK + 63) / 64; ++w) s += red[w];
       part[tile] = s;
     }
+  }
+}
+
+// ---- multi-output forward z pass.  The three components of an operator
+// that shares its inputs (the stress) are transformed in one tile: its TK
+// pencils form MO = 4 groups of R = TK/4 rows, group o holding component o
+// of rows row0 .. row0+R-1 (group 3 idles).  The load phase reads every input
+// element once and writes the three values; the transform runs on all TK
+// pencils; the store phase writes group o to outs[o].  dmask bit o: DST along
+// z for component o.
+constexpr int kMultiGroups = 4;
+
+template <typename T, int N>
+constexpr bool multi_ok() {
+  return TileCfg<T, N, kAxZ, 0>::TK % kMultiGroups == 0;
+}
+
+template <typename T, int N>
+__host__ __device__ inline long multi_tiles(const Geo& g) {
+  constexpr int R = TileCfg<T, N, kAxZ, 0>::TK / kMultiGroups;
+  return ((long)g.nx * g.ny + R - 1) / R;
+}
+
+template <typename T, int N, int LD, class SM>
+__device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int tile, SM& sm,
+                                                const int tid) {
+  using C = TileCfg<T, N, kAxZ, 0>;
+  using PL = Plan<N / 2>;
+  constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
+  constexpr int R1 = PL::R1;
+  constexpr int MO = kMultiGroups, R = TK / MO, NO = multi_count<LD>();
+  static_assert(TK % MO == 0 && NO <= MO, "multi-output tile");
+  using TW = TwLayout<M>;
+  cx<T>* const buf = sm.buf;
+  cx<T>* const ts = sm.ts;
+  cx<T>* const tp = sm.tp;
+  T* rbuf = (T*)buf;
+
+  {
+    const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
+    for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
+    for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
+  }
+  // the walk over the tile's R rows (element s of a thread: row q, sample n)
+  TileGeo<kAxZ, N, R, BLOCK> tg;
+  tg.init(A.geo, tile, tid);
+  constexpr int PER = R * N / BLOCK;
+  constexpr bool SPLIT = N % BLOCK == 0;
+  constexpr int RZ = SPLIT ? N / BLOCK : 1;
+  auto row_ok = [&](int s) { return tg.row0 + s / RZ < tg.nrows; };
+  {
+    T xs[PER][NO];
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      int q, n, i, j, k;
+      Ix idx;
+      bool ok;
+      tg.walk(tid, s, q, n, i, j, k, idx, ok);
+      if constexpr (SPLIT) idx = tg.wbase + (Ix)(row_ok(s) ? s * BLOCK : (s % RZ) * BLOCK);
+      multi_eval<T, LD>(A.ld, A.phys, idx, xs[s]);
+    }
+    // Makhoul order; component o of row q in pencil o R + q
+    WalkPos<kAxZ, N, R, TKP, BLOCK, 1> wp;
+    wp.init(tid);
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      bool oddn;
+      const int lp = wp.pos(tid, s, oddn);
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        T x = xs[s][o];
+        if (((A.dmask >> o) & 1) & oddn) x = -x;
+        rbuf[lp + 2 * o * R] = x;
+      }
+    }
+  }
+  __syncthreads();
+  const int p = tid % TK, t = tid / TK;
+  const bool dst = (A.dmask >> (p / R)) & 1;  // this thread's pencil group
+  cx<T> v[E];
+  gather<T, M, E, TKP, R1>(v, buf, p, t);
+  run_fft<T, N, M, E, TKP, -1, true>(typename PL::R{}, v, buf, p, t, ts);
+
+  // DCT-II post-processing (as in pass_tile), staged in LDS
+  constexpr int NPRE = E / 2;
+  T y0[NPRE + 1], y1[NPRE + 1], y2[NPRE + 1], y3[NPRE + 1];
+#pragma unroll
+  for (int b = 0; b <= NPRE; ++b) {
+    int kk = t + b * TP;
+    if (b == NPRE) kk = (t == 0) ? M / 2 : -1;
+    if (kk < 0) continue;
+    const cx<T> a = buf[kk * TKP + p], c = conj(buf[(kk ? M - kk : 0) * TKP + p]);
+    const cx<T> Ev = a + c;
+    const cx<T> Ov = mul_si<-1>(a - c);
+    const cx<T> wO = cmul(tp[3 * kk], Ov);
+    const cx<T> Wk = cmul(tp[3 * kk + 1], Ev + wO);
+    const cx<T> Wm = cmul(tp[3 * kk + 2], conj(Ev - wO));
+    y0[b] = Wk.x; y1[b] = -Wk.y; y2[b] = Wm.x; y3[b] = -Wm.y;
+  }
+  PencilPos<TKP> pp;
+  pp.init(p, t);
+  auto put_k = [&](bool up, int K, T val) {
+    const int pa = up ? pp.up(K) : pp.dn(K);
+    const int pb = up ? pp.dn(N - 1 - K) : pp.up(N - 1 - K);
+    rbuf[dst ? pb : pa] = val;
+  };
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NPRE; ++b) {
+    const int C0 = b * TP;
+    put_k(true, C0, y0[b]);
+    if (C0 > 0 || t > 0) put_k(false, N - C0, y1[b]);
+    put_k(false, M - C0, y2[b]);
+    if (C0 > 0 || t > 0) put_k(true, M + C0, y3[b]);
+  }
+  if (t == 0) {
+    const int n0 = dst ? N - 1 - M / 2 : M / 2, n1 = dst ? M / 2 - 1 : N - M / 2;
+    rbuf[lpos<TKP>(n0, p)] = y0[NPRE];
+    rbuf[lpos<TKP>(n1, p)] = y1[NPRE];
+  }
+  __syncthreads();
+  WalkPos<kAxZ, N, R, TKP, BLOCK, 0> wq;
+  wq.init(tid);
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    bool oddn;
+    int q, n, i, j, k;
+    Ix idx;
+    bool ok;
+    tg.walk(tid, s, q, n, i, j, k, idx, ok);
+    if constexpr (SPLIT) ok = row_ok(s);
+    if (!ok) continue;
+    const int lp = wq.pos(tid, s, oddn);
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      if constexpr (SPLIT) st_at<T>(A.out[o], tg.wbase, s * BLOCK, rbuf[lp + 2 * o * R]);
+      else st<T>(A.out[o], idx, rbuf[lp + 2 * o * R]);
+    }
+  }
+}
+
+template <typename T, int N, int LD>
+__global__ void __launch_bounds__((TileCfg<T, N, kAxZ, 0>::BLOCK)) k_multi(const MultiArgs A) {
+  __shared__ Smem<T, N, kAxZ, false, 0, 0> sm;
+  const int nb = (int)gridDim.x;
+  pass_tile_multi<T, N, LD>(A, nb - 1 - (int)blockIdx.x, sm, (int)threadIdx.x);
+}
+
+template <typename T, int N>
+hipError_t launch_multi_n(const MultiArgs& A, long* nblocks, hipStream_t s) {
+  if constexpr (!multi_ok<T, N>()) {
+    return hipErrorInvalidValue;
+  } else {
+    const long nb = multi_tiles<T, N>(A.geo);
+    if (nblocks) *nblocks = nb;
+    if (nb <= 0) return hipSuccess;
+    constexpr int BLOCK = TileCfg<T, N, kAxZ, 0>::BLOCK;
+    if (A.ld.code == LD_STRESS3)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS_STB3)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
   }
 }
 
@@ -768,42 +934,35 @@ struct GroupOps<GRP_PSI> {
                    LD_LAPGRAD_Y, LD_LAPGRAD_Z>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
+  static constexpr bool XORD = false;
 };
 template <>
 struct GroupOps<GRP_RHO> {
   using L = OpList<LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
+  static constexpr bool XORD = false;
 };
 template <>
 struct GroupOps<GRP_PROJ> {
   using L = OpList<LD_PRESSURE, LD_VEL0, LD_VEL1, LD_VEL2>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
-};
-template <>
-struct GroupOps<GRP_STRESS> {
-  using L = OpList<LD_STRESS, LD_STRESS, LD_STRESS>;
-  static constexpr bool INV = false;
-  static constexpr int AX = kAxZ;
+  static constexpr bool XORD = true;
 };
 template <>
 struct GroupOps<GRP_STB_PSI> {
   using L = OpList<LD_SH, LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
+  static constexpr bool XORD = false;
 };
 template <>
 struct GroupOps<GRP_STB_PROJ> {
   using L = OpList<LD_VEL0, LD_VEL1, LD_VEL2>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
-};
-template <>
-struct GroupOps<GRP_STB_STRESS> {
-  using L = OpList<LD_STRESS_STB, LD_STRESS_STB, LD_STRESS_STB>;
-  static constexpr bool INV = false;
-  static constexpr int AX = kAxZ;
+  static constexpr bool XORD = true;
 };
 
 template <int AX, int... LDs>
@@ -825,7 +984,10 @@ __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& 
                                           SM& sm) {
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0>(G.geo, G.waves, G.phys, G.out[O].ld, G.out[O].st,
+  // inverse (x) groups store with non-temporal hints: their outputs are read
+  // again only by the next passes, after the whole group, so they should not
+  // evict the group's inputs, which every output re-reads
+  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, AX == kAxX>(G.geo, G.waves, G.phys, G.out[O].ld, G.out[O].st,
                                             G.tw, nullptr, G.out[O].dst, tile, sm, O == 0, tid);
   if constexpr (sizeof...(REST) > 0) {
     __syncthreads();  // the next pass rewrites the tile in LDS
@@ -861,7 +1023,10 @@ __global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
   using GO = GroupOps<GRP>;
   constexpr int AX = GO::AX;
   __shared__ GroupSmem<T, N, GRP> sm;
-  const int tile = tile_order<T, N, AX, 0>(G.geo, (int)blockIdx.x, (int)gridDim.x);
+  // projection groups read the j-1 and j+1 neighbours of their rows: one
+  // contiguous range of rows per XCD keeps those in the XCD's L2
+  const int tile = GO::XORD ? xcd_tile((int)blockIdx.x, (int)gridDim.x)
+                            : tile_order<T, N, AX, 0>(G.geo, (int)blockIdx.x, (int)gridDim.x);
   run_group<T, N, AX, GO::INV, decltype(sm), 0>(typename GO::L{}, G, tile, sm);
 }
 
@@ -912,10 +1077,8 @@ hipError_t launch_group_n(int group, const GroupArgs& G, long* nblocks, hipStrea
     case GRP_PSI: return launch_group_g<T, N, AX, GRP_PSI>(G, nblocks, s);
     case GRP_RHO: return launch_group_g<T, N, AX, GRP_RHO>(G, nblocks, s);
     case GRP_PROJ: return launch_group_g<T, N, AX, GRP_PROJ>(G, nblocks, s);
-    case GRP_STRESS: return launch_group_g<T, N, AX, GRP_STRESS>(G, nblocks, s);
     case GRP_STB_PSI: return launch_group_g<T, N, AX, GRP_STB_PSI>(G, nblocks, s);
     case GRP_STB_PROJ: return launch_group_g<T, N, AX, GRP_STB_PROJ>(G, nblocks, s);
-    case GRP_STB_STRESS: return launch_group_g<T, N, AX, GRP_STB_STRESS>(G, nblocks, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -29,7 +29,7 @@ hipError_t QSM_CAT(launch_pass, QSM_T, QSM_AX)(int n, bool inv, int ld, int st, 
   }
 }
 
-#if QSM_AX == 0 || QSM_AX == 2
+#if QSM_AX == 0
 template <int N, bool STEP>
 static hipError_t launch_group_if(int group, const GroupArgs& G, long* nb, hipStream_t s) {
   if constexpr (STEP) return launch_group_n<QSM_T, N, QSM_AX>(group, G, nb, s);
@@ -46,6 +46,38 @@ hipError_t QSM_CAT(launch_group, QSM_T, QSM_AX)(int n, int group, const GroupArg
 #undef QSM_CASE
     default:
       return hipErrorInvalidValue;
+  }
+}
+#endif
+
+#if QSM_AX == 2
+template <int N, bool STEP>
+static hipError_t launch_multi_if(const MultiArgs& A, long* nb, hipStream_t s) {
+  if constexpr (STEP) return launch_multi_n<QSM_T, N>(A, nb, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t QSM_CAT(launch_multi, QSM_T, n)(int n, const MultiArgs& A, long* nb, hipStream_t s) {
+  switch (n) {
+#define QSM_CASE(NN) \
+  case NN:           \
+    return launch_multi_if<NN, QSM_IS_STEP_N(NN)>(A, nb, s);
+    QSM_FOR_EACH_N(QSM_CASE)
+#undef QSM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+bool QSM_CAT(multi_ok, QSM_T, n)(int n) {
+  switch (n) {
+#define QSM_CASE(NN) \
+  case NN:           \
+    return QSM_IS_STEP_N(NN) && multi_ok<QSM_T, NN>();
+    QSM_FOR_EACH_N(QSM_CASE)
+#undef QSM_CASE
+    default:
+      return false;
   }
 }
 #endif
