@@ -42,7 +42,16 @@ SWEEPS_PER_STEP = 194
 LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "lapgrad_x",
             "lapgrad_y", "lapgrad_z", "sh", "rhostar", "divv",
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
-            "stress_stb", "nonlin_stb", "nonlin_stb0"]
+            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3"]
+# timer categories 40 + g of the x-pass group kernels (one kernel, several outputs)
+GROUPS = ["psi", "rho", "proj", "stb_psi", "stb_proj"]
+
+
+def load_op_name(code):
+    if code < len(LOAD_OPS):
+        return LOAD_OPS[code]
+    g = code - 40
+    return f"group {GROUPS[g]}" if 0 <= g < len(GROUPS) else f"op {code}"
 STORE_OPS = ["plain", "filter", "update", "update_stb"]
 
 
@@ -195,8 +204,14 @@ def main():
     value = pts * args.steps / m["s"]
     k = top
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
-    symbol = (f"void qsm::k_pass<float, {shape[0]}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
-              f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
+    n_ax = shape[k["axis"]]
+    if k["ld"] >= 40:  # x-pass group kernel
+        symbol = f"void qsm::k_group<float, {n_ax}, {k['ld'] - 40}>(qsm::GroupArgs)"
+    elif LOAD_OPS[k["ld"]] in ("stress3", "stress_stb3"):  # multi-output z pass
+        symbol = f"void qsm::k_multi<float, {n_ax}, {k['ld']}>(qsm::MultiArgs)"
+    else:
+        symbol = (f"void qsm::k_pass<float, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+                  f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
     direction = "inverse" if k["inv"] else "forward"
     out = {
         "metric": "grid-pt updates/s at 512^3, mixed vs FP64 on 1 GPU; % HBM roofline; psi rel-L2",
@@ -217,7 +232,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(symbol),
                      "kernel": symbol,
-                     "pass": f"{direction} {'xyz'[k['axis']]} pass, load op {LOAD_OPS[k['ld']]}, "
+                     "pass": f"{direction} {'xyz'[k['axis']]} pass, load op {load_op_name(k['ld'])}, "
                              f"store op {STORE_OPS[k['st']]}",
                      "avg_launch_ms": k["ms"] / max(1, k["launches"]),
                      "launches": k["launches"],

@@ -477,10 +477,12 @@ int run_group_first(qsm_handle_s* h, int group, const GroupItem* it, int count, 
   return QSM_OK;
 }
 
-// inverse transforms x, y, z (the x pass as one group kernel)
+// inverse transforms x, y, z (the x pass as one group kernel).  The y and z
+// passes take the outputs last-written first: the group kernel's last tiles
+// are still in the memory-side cache when the next y pass starts on them.
 int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
   int rc = run_group_first(h, group, it, count, reads);
-  for (int o = 0; o < count && !rc; ++o) {
+  for (int o = count - 1; o >= 0 && !rc; --o) {
     rc = run_pass(h, 1, true, it[o].dst_axis == 1, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
     if (!rc)
       rc = run_pass(h, 2, true, it[o].dst_axis == 2, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);

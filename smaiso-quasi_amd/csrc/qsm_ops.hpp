@@ -339,6 +339,72 @@ __device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phy
   }
 }
 
+// Real-space operators of the forward z passes as functions of their input
+// values: in[f] is field a[f] at the element.  load_eval reads the inputs
+// and applies the same function, so element-wise and vectorized loads give
+// the same bits.
+template <int LD>
+constexpr int op_nin() {
+  return LD == LD_PLAIN ? 1 : LD == LD_RHOSTAR ? 4 : LD == LD_DIVV ? 8 : LD == LD_STRESS ? 5
+       : LD == LD_NONLIN ? 10 : LD == LD_NONLIN_P ? 14 : LD == LD_STRESS_STB ? 3
+       : LD == LD_NONLIN_STB ? 7 : LD == LD_NONLIN_STB0 ? 1 : 0;
+}
+template <typename T, int LD>
+__device__ __forceinline__ T op_apply(const T* in, const Phys& P) {
+  if constexpr (LD == LD_PLAIN) {
+    return in[0];
+  } else if constexpr (LD == LD_RHOSTAR) {
+    const T ps = in[0], gx = in[1], gy = in[2], gz = in[3];
+    return (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
+  } else if constexpr (LD == LD_DIVV) {
+    // rho, rho_old, vx, vy, vz, rx, ry, rz
+    const T r = in[0], ro = in[1];
+    T t = (r - ro) / (T)P.dt + in[2] * in[5];
+    t = t + in[3] * in[6];
+    t = t + in[4] * in[7];
+    return -t / ((T)0.5 * (r + ro));
+  } else if constexpr (LD == LD_STRESS) {
+    // rho, G, F, g_c, l_c
+    const T r = in[0];
+    const T gg = r * in[1];
+    const T rf = r * in[2];
+    return gg * in[3] - rf * in[4];
+  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P) {
+    // rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
+    const T r = in[0], ps = in[1];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    T v = r * ((T)P.eps * ps - in[2] + (T)P.beta * ps3 - (T)P.gamma * ps5) - r * in[3];
+    v = v - in[4] * in[7];
+    v = v - in[5] * in[8];
+    v = v - in[6] * in[9];
+    if constexpr (LD == LD_NONLIN_P) {
+      const T rx = in[11], ry = in[12], rz = in[13];
+      v = v + in[10] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) / (T)P.rho_s2;
+    }
+    return v;
+  } else if constexpr (LD == LD_STRESS_STB) {
+    // sh, psi, g_c ; rho = 1
+    const T ps = in[1];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    const T mu = in[0] - (T)P.beta * ps3 + (T)P.gamma * ps5;
+    return mu * in[2];
+  } else if constexpr (LD == LD_NONLIN_STB || LD == LD_NONLIN_STB0) {
+    // psi, vx, vy, vz, gx, gy, gz
+    const T ps = in[0];
+    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
+    T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
+    if constexpr (LD == LD_NONLIN_STB) {
+      v = v - in[1] * in[4];
+      v = v - in[2] * in[5];
+      v = v - in[3] * in[6];
+    }
+    return v;
+  } else {
+    static_assert(op_nin<LD>() > 0, "not a real-space operator");
+    return (T)0;
+  }
+}
+
 // K carries the element's wavenumbers for the x-pass spectral operators.
 template <typename T, int LD>
 __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv& K,
@@ -393,60 +459,21 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
     const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
     const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
     return (T)(-alin * (double)ld<T>(L.a[0], idx));
-  } else if constexpr (LD == LD_RHOSTAR) {
-    const T ps = ld<T>(L.a[0], idx), gx = ld<T>(L.a[1], idx),
-            gy = ld<T>(L.a[2], idx), gz = ld<T>(L.a[3], idx);
-    return (T)P.kp * sqrt((T)P.q02 * (ps * ps) + gx * gx + gy * gy + gz * gz) + (T)P.rho0;
-  } else if constexpr (LD == LD_DIVV) {
-    // a: rho, rho_old, vx, vy, vz, rx, ry, rz
-    const T r = ld<T>(L.a[0], idx), ro = ld<T>(L.a[1], idx);
-    T t = (r - ro) / (T)P.dt + ld<T>(L.a[2], idx) * ld<T>(L.a[5], idx);
-    t = t + ld<T>(L.a[3], idx) * ld<T>(L.a[6], idx);
-    t = t + ld<T>(L.a[4], idx) * ld<T>(L.a[7], idx);
-    return -t / ((T)0.5 * (r + ro));
-  } else if constexpr (LD == LD_STRESS) {
-    // a: rho, G, F, g_c, l_c
-    const T r = ld<T>(L.a[0], idx);
-    const T gg = r * ld<T>(L.a[1], idx);
-    const T rf = r * ld<T>(L.a[2], idx);
-    return gg * ld<T>(L.a[3], idx) - rf * ld<T>(L.a[4], idx);
+  } else if constexpr (LD == LD_RHOSTAR || LD == LD_DIVV || LD == LD_STRESS) {
+    T in[op_nin<LD>()];
+#pragma unroll
+    for (int f = 0; f < op_nin<LD>(); ++f) in[f] = ld<T>(L.a[f], idx);
+    return op_apply<T, LD>(in, P);
   } else if constexpr (LD == LD_PRESSURE || LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2) {
     return finish<T, LD>(fetch<T, LD>(L, G, i, j, k, idx), L, P, K, G, i, j, k);
   } else if constexpr (LD == LD_SCALE) {
     return (T)(P.scale * rd<T>(L.a[0], idx));
-  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P) {
-    // a: rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
-    const T r = ld<T>(L.a[0], idx), ps = ld<T>(L.a[1], idx);
-    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    T v = r * ((T)P.eps * ps - ld<T>(L.a[2], idx) + (T)P.beta * ps3 - (T)P.gamma * ps5) -
-          r * ld<T>(L.a[3], idx);
-    v = v - ld<T>(L.a[4], idx) * ld<T>(L.a[7], idx);
-    v = v - ld<T>(L.a[5], idx) * ld<T>(L.a[8], idx);
-    v = v - ld<T>(L.a[6], idx) * ld<T>(L.a[9], idx);
-    if constexpr (LD == LD_NONLIN_P) {
-      const T rx = ld<T>(L.a[11], idx), ry = ld<T>(L.a[12], idx),
-              rz = ld<T>(L.a[13], idx);
-      v = v + ld<T>(L.a[10], idx) * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) /
-                  (T)P.rho_s2;
-    }
-    return v;
-  } else if constexpr (LD == LD_STRESS_STB) {
-    // a: sh, psi, g_c ; rho = 1
-    const T ps = ld<T>(L.a[1], idx);
-    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    const T mu = ld<T>(L.a[0], idx) - (T)P.beta * ps3 + (T)P.gamma * ps5;
-    return mu * ld<T>(L.a[2], idx);
-  } else if constexpr (LD == LD_NONLIN_STB || LD == LD_NONLIN_STB0) {
-    // a: psi, vx, vy, vz, gx, gy, gz
-    const T ps = ld<T>(L.a[0], idx);
-    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    T v = (T)P.beta * ps3 - (T)P.gamma * ps5;
-    if constexpr (LD == LD_NONLIN_STB) {
-      v = v - ld<T>(L.a[1], idx) * ld<T>(L.a[4], idx);
-      v = v - ld<T>(L.a[2], idx) * ld<T>(L.a[5], idx);
-      v = v - ld<T>(L.a[3], idx) * ld<T>(L.a[6], idx);
-    }
-    return v;
+  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P || LD == LD_STRESS_STB ||
+                       LD == LD_NONLIN_STB || LD == LD_NONLIN_STB0) {
+    T in[op_nin<LD>()];
+#pragma unroll
+    for (int f = 0; f < op_nin<LD>(); ++f) in[f] = ld<T>(L.a[f], idx);
+    return op_apply<T, LD>(in, P);
   } else {
     static_assert(LD == LD_PLAIN, "unknown load operator");
     return (T)0;
@@ -463,18 +490,21 @@ constexpr int multi_count() {
 template <typename T, int LD>
 __device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx, T (&out)[3]) {
   if constexpr (LD == LD_STRESS3) {
-    const T r = ld<T>(L.a[0], idx);
-    const T gg = r * ld<T>(L.a[1], idx);
-    const T rf = r * ld<T>(L.a[2], idx);
+    T in[5] = {ld<T>(L.a[0], idx), ld<T>(L.a[1], idx), ld<T>(L.a[2], idx), (T)0, (T)0};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] = gg * ld<T>(L.a[3 + c], idx) - rf * ld<T>(L.a[6 + c], idx);
+    for (int c = 0; c < 3; ++c) {
+      in[3] = ld<T>(L.a[3 + c], idx);
+      in[4] = ld<T>(L.a[6 + c], idx);
+      out[c] = op_apply<T, LD_STRESS>(in, P);
+    }
   } else {
     static_assert(LD == LD_STRESS_STB3, "not a multi-output operator");
-    const T ps = ld<T>(L.a[1], idx);
-    const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
-    const T mu = ld<T>(L.a[0], idx) - (T)P.beta * ps3 + (T)P.gamma * ps5;
+    T in[3] = {ld<T>(L.a[0], idx), ld<T>(L.a[1], idx), (T)0};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] = mu * ld<T>(L.a[2 + c], idx);
+    for (int c = 0; c < 3; ++c) {
+      in[2] = ld<T>(L.a[2 + c], idx);
+      out[c] = op_apply<T, LD_STRESS_STB>(in, P);
+    }
   }
 }
 
