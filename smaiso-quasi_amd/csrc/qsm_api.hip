@@ -490,13 +490,14 @@ int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int r
   return rc;
 }
 
-// Forward transforms z, y, x of the three components of a multi-output
-// operator (L: LD_STRESS3 / LD_STRESS_STB3, component c with DST along c):
-// the z pass of all three as one kernel reading the shared inputs once
-// (launch_multi), then y and x in place per component.  `reads`: distinct
-// fields the operator reads (algorithmic bytes).  Returns 1 (no error set)
-// where the pencil length has no multi-output kernel.
-int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const out[3], int reads) {
+// Forward transforms z, y, x of the nout values of a multi-output operator
+// (L: LD_STRESS3 / LD_STRESS_STB3 / LD_STRESS3_DIVV; value c < 3 is the
+// stress component c, with its DST along c; value 3 the divergence, DCT on
+// every axis): the z pass of all of them as one kernel reading the shared
+// inputs once (launch_multi), then y and x in place per value.  `reads`:
+// distinct fields the operator reads (algorithmic bytes).  Returns 1 (no
+// error set) where the pencil length has no multi-output kernel.
+int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int reads) {
   if (!multi_supported(h->fp32, h->p.nz)) return 1;
   if (!h->dry) {
     if (h->rec) return fail(QSM_ERR_STATE, "multi-output passes cannot be recorded");
@@ -506,7 +507,7 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const out[3], int reads) 
     A.ld = L;
     A.tw = h->tw[2];
     A.dmask = 1 << 2;  // component z: DST along z
-    for (int c = 0; c < 3; ++c) A.out[c] = out[c];
+    for (int c = 0; c < nout; ++c) A.out[c] = out[c];
     Timer& T = h->timer;
     hipEvent_t ea = nullptr, eb = nullptr;
     if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
@@ -515,10 +516,10 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const out[3], int reads) 
     if (T.on) {
       hipEventRecord(eb, h->stream);
       T.recs.push_back({ea, eb, timer_key(2, false, L.code, ST_PLAIN, 0),
-                        (double)(reads + 3) * (double)h->n * (double)h->esz});
+                        (double)(reads + nout) * (double)h->n * (double)h->esz});
     }
   }
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < nout; ++c) {
     int rc = run_pass(h, 1, false, c == 1, plain_load(out[c]), plain_store(out[c]), false, nullptr);
     if (!rc) rc = run_pass(h, 0, false, c == 0, plain_load(out[c]), plain_store(out[c]), false, nullptr);
     if (rc) return rc;
@@ -599,8 +600,11 @@ int quasi_step(qsm_handle_s* h) {
   // S4: divergence (quasi.cpp:1273-1293).  Up to divv_switch the reference
   // transforms an all-zero D; D-hat = 0 then, which the D-hat array already
   // holds (it is zeroed at set-up and first written at step divv_switch+1),
-  // so the transform of zeros is not run.
-  if (dlive) {
+  // so the transform of zeros is not run.  S4 and S5 read only fields of
+  // earlier stages: single handles run D (when live) as the fourth value of
+  // the stress components' multi-output z pass.
+  const bool multi = grp && multi_supported(h->fp32, p.nz);
+  if (dlive && !multi) {
     L = LoadOp{}; L.code = LD_DIVV;
     L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
     L.a[5] = F(h, F_RX); L.a[6] = F(h, F_RY); L.a[7] = F(h, F_RZ);
@@ -608,15 +612,15 @@ int quasi_step(qsm_handle_s* h) {
   }
 
   // S5: stress (quasi.cpp:1298-1330)
-  rc = 1;
-  if (grp) {
-    L = LoadOp{}; L.code = LD_STRESS3;
+  if (multi) {
+    L = LoadOp{}; L.code = dlive ? LD_STRESS3_DIVV : LD_STRESS3;
     L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F);
     for (int c = 0; c < 3; ++c) { L.a[3 + c] = F(h, F_GX + c); L.a[6 + c] = F(h, F_LX + c); }
-    void* const out[3] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ)};
-    if ((rc = fwd3_multi(h, L, out, 9)) < 0) return rc;
-  }
-  if (rc) {
+    L.a[9] = rho_old;
+    for (int c = 0; c < 3; ++c) { L.a[10 + c] = F(h, F_VX + c); L.a[13 + c] = F(h, F_RX + c); }
+    void* const out[4] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ), F(h, F_DHAT)};
+    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9))) return rc;
+  } else {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
       L.a[0] = rho; L.a[1] = F(h, F_G); L.a[2] = F(h, F_F); L.a[3] = F(h, F_GX + c); L.a[4] = F(h, F_LX + c);
@@ -707,7 +711,7 @@ int stb_step(qsm_handle_s* h) {
     L.a[0] = F(h, F_F); L.a[1] = psi;
     for (int c = 0; c < 3; ++c) L.a[2 + c] = F(h, F_GX + c);
     void* const out[3] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ)};
-    if ((rc = fwd3_multi(h, L, out, 5)) == 1) {
+    if ((rc = fwd3_multi(h, L, out, 3, 5)) == 1) {
       for (int c = 0; c < 3; ++c) {
         L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
         L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);

@@ -80,15 +80,15 @@ struct GroupArgs {
   const void* tw;  // twiddle table of the group's axis
   GroupOut out[kMaxGroup];
 };
-// Multi-output forward z pass (the three stress components from one read
-// of their shared inputs): ld is a multi-output operator (LD_STRESS3,
-// LD_STRESS_STB3), out[c] receives component c, dmask bit c selects DST
-// along z for component c.
+// Multi-output forward z pass (the three stress components, and the
+// divergence, from one read of their shared inputs): ld is a multi-output
+// operator (LD_STRESS3, LD_STRESS_STB3, LD_STRESS3_DIVV), out[c] receives
+// value c, dmask bit c selects DST along z for value c.
 struct MultiArgs {
   Geo geo;
   Phys phys;
   LoadOp ld;
-  void* out[3];
+  void* out[4];
   const void* tw;  // z-axis twiddle table
   int dmask;
   int pad_;

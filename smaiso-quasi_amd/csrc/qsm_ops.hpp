@@ -64,6 +64,8 @@ enum LoadCode : int {
   // multi-output operators (three values per element, one per component c)
   LD_STRESS3,       // LD_STRESS for c = x, y, z; a: rho, G, F, g_x,y,z, l_x,y,z
   LD_STRESS_STB3,   // LD_STRESS_STB for c = x, y, z; a: sh, psi, g_x,y,z
+  LD_STRESS3_DIVV,  // LD_STRESS3, then LD_DIVV as a fourth value; a: as LD_STRESS3, then
+                    //   rho_old, v_x,y,z, r_x,y,z (rho is a0)
 };
 
 enum StoreCode : int {
@@ -78,7 +80,7 @@ struct LoadOp {
   int axis;      // component c (informative: STRESS)
   int flag;      // unused (reserved)
   int n0;        // VEL0-2: slab size of the P-rank halo compat map (0 = exact)
-  const void* a[14];
+  const void* a[16];
 };
 
 struct StoreOp {
@@ -485,17 +487,24 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
 // the single-output operator (LD_STRESS, LD_STRESS_STB) term for term.
 template <int LD>
 constexpr int multi_count() {
-  return (LD == LD_STRESS3 || LD == LD_STRESS_STB3) ? 3 : 1;
+  return LD == LD_STRESS3_DIVV ? 4 : ((LD == LD_STRESS3 || LD == LD_STRESS_STB3) ? 3 : 1);
 }
 template <typename T, int LD>
-__device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx, T (&out)[3]) {
-  if constexpr (LD == LD_STRESS3) {
+__device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx,
+                                           T (&out)[multi_count<LD>()]) {
+  if constexpr (LD == LD_STRESS3 || LD == LD_STRESS3_DIVV) {
     T in[5] = {ld<T>(L.a[0], idx), ld<T>(L.a[1], idx), ld<T>(L.a[2], idx), (T)0, (T)0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       in[3] = ld<T>(L.a[3 + c], idx);
       in[4] = ld<T>(L.a[6 + c], idx);
       out[c] = op_apply<T, LD_STRESS>(in, P);
+    }
+    if constexpr (LD == LD_STRESS3_DIVV) {
+      T d[8] = {in[0]};
+#pragma unroll
+      for (int f = 1; f < 8; ++f) d[f] = ld<T>(L.a[8 + f], idx);
+      out[3] = op_apply<T, LD_DIVV>(d, P);
     }
   } else {
     static_assert(LD == LD_STRESS_STB3, "not a multi-output operator");

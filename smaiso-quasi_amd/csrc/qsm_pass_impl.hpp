@@ -718,13 +718,13 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   }
 }
 
-// ---- multi-output forward z pass.  The three components of an operator
-// that shares its inputs (the stress) are transformed in one tile: its TK
-// pencils form MO = 4 groups of R = TK/4 rows, group o holding component o
-// of rows row0 .. row0+R-1 (group 3 idles).  The load phase reads every input
-// element once and writes the three values; the transform runs on all TK
-// pencils; the store phase writes group o to outs[o].  dmask bit o: DST along
-// z for component o.
+// ---- multi-output forward z pass.  The NO = 3 or 4 values of an operator
+// that shares its inputs (the stress components, and the divergence when it
+// is live) are transformed in one tile: its TK pencils form MO = 4 groups of
+// R = TK/4 rows, group o holding value o of rows row0 .. row0+R-1 (with three
+// values, group 3 idles).  The load phase reads every input element once and
+// writes the NO values; the transform runs on all TK pencils; the store
+// phase writes group o to A.out[o].  dmask bit o: DST along z for value o.
 constexpr int kMultiGroups = 4;
 
 template <typename T, int N>
@@ -766,7 +766,7 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   constexpr int RZ = SPLIT ? N / BLOCK : 1;
   auto row_ok = [&](int s) { return tg.row0 + s / RZ < tg.nrows; };
   {
-    T xs[PER][NO];
+    T xs[PER][NO];  // NO values of each element
 #pragma unroll
     for (int s = 0; s < PER; ++s) {
       int q, n, i, j, k;
@@ -876,6 +876,8 @@ hipError_t launch_multi_n(const MultiArgs& A, long* nblocks, hipStream_t s) {
       hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
     else if (A.ld.code == LD_STRESS_STB3)
       hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS3_DIVV)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3_DIVV>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
     else
       return hipErrorInvalidValue;
     return hipGetLastError();
