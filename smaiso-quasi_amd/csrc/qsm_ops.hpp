@@ -241,13 +241,13 @@ __device__ __forceinline__ Shifts shifts(const LoadOp& L, const Geo& G, int i, i
     // zero at j = Ny-1 or k = Nz-1.  The S_x neighbour slab is i-1, or the
     // P-rank halo slab (quasi.cpp:1639-1650).
     const bool in = (j < G.ny - 1) & (k < G.nz - 1);
-    // i mod n0 without an integer division: floor((i + 1/2)/n0) in FP32 is
-    // exact for i < 2^22 (relative error 2^-23 against a margin of 1/(2 n0))
-    const int n0 = L.n0 > 0 ? L.n0 : 1;
-    const int qn = (int)(((float)i + 0.5f) * (1.0f / (float)n0));
-    const bool halo = (L.n0 > 0) & (i > 0) & (i == qn * n0);
-    const int ih = halo ? i - L.n0 : i - 1;
-    const int dx = (ih - i) * nyz;
+    int dx = -nyz;  // exact stencil: slab i-1
+    if (L.n0 > 0) {  // uniform: the P-rank halo map
+      // i mod n0 without an integer division: floor((i + 1/2)/n0) in FP32 is
+      // exact for i < 2^22 (relative error 2^-23 against a margin of 1/(2 n0))
+      const int qn = (int)(((float)i + 0.5f) * (1.0f / (float)L.n0));
+      if ((i > 0) & (i == qn * L.n0)) dx = -L.n0 * nyz;
+    }
     if constexpr (LD == LD_VEL1)  // a: S_y, b: S_x[ih][j+1], c: S_z[j+1][k-1], d: D[j+1]
       return Shifts{true, band(in, i > 0), band(in, k > 0), band(in, hasd), 0, dx + G.nz, G.nz - 1, G.nz};
     else  // a: S_z, b: S_x[ih][k+1], c: S_y[j-1][k+1], d: D[k+1]
@@ -285,7 +285,8 @@ __device__ __forceinline__ float finish32(const Raw4<float>& r, const LoadOp& L,
   const float nu = (float)P.nu, scale = (float)P.scale;
   if constexpr (LD == LD_PRESSURE) {
     const float qx = (float)K.qx, qy = (float)K.qy, qz = (float)K.qz;
-    const float rm2 = __builtin_amdgcn_rcpf(qx * qx + qy * qy + qz * qz);
+    // these operators run on the x pass: the y, z terms are per thread
+    const float rm2 = __builtin_amdgcn_rcpf(qx * qx + (qy * qy + qz * qz));
     const float pv = -scale * (qx * a + qy * b + qz * c) * rm2;
     return (i + j + k == 0) ? 0.0f : pv;
   } else {
@@ -293,7 +294,7 @@ __device__ __forceinline__ float finish32(const Raw4<float>& r, const LoadOp& L,
     constexpr int cc = LD == LD_VEL0 ? 0 : (LD == LD_VEL1 ? 1 : 2);
     const float kx = (float)(cc == 0 ? K.sx : K.qx), ky = (float)(cc == 1 ? K.sy : K.qy),
                 kz = (float)(cc == 2 ? K.sz : K.qz);
-    const float rm2 = __builtin_amdgcn_rcpf(kx * kx + ky * ky + kz * kz);
+    const float rm2 = __builtin_amdgcn_rcpf(kx * kx + (ky * ky + kz * kz));
     const float ks = cc == 0 ? kx : (cc == 1 ? ky : kz);
     // read a is the own component; b, c the others in axis order
     const float dot = cc == 0 ? kx * a + ky * b + kz * c
