@@ -268,7 +268,8 @@ using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<A
 
 // One workgroup's work: tile `tile` of the pass described by A, in the LDS
 // sm.  stage = false skips the staging of twiddles and tables a previous
-// pass of the same kernel (same axis and direction) left in sm.
+// pass of the same kernel (same axis and direction) left in sm.  NTS: the
+// direct stores of inverse strided passes are non-temporal.
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM, bool NTS = false>
 __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Phys& Ph,
                                           const LoadOp& Lop, const StoreOp& Sop,
@@ -925,7 +926,8 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
       tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
 }
 
-// ---- group passes (z axis): the operator list of each group
+// ---- group passes (the x pass of inverse transforms): the operators of
+// each group in output order
 template <int... LDs>
 struct OpList {};
 template <int G>
@@ -986,11 +988,12 @@ __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& 
                                           SM& sm) {
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  // inverse (x) groups store with non-temporal hints: their outputs are read
-  // again only by the next passes, after the whole group, so they should not
-  // evict the group's inputs, which every output re-reads
-  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, AX == kAxX>(G.geo, G.waves, G.phys, G.out[O].ld, G.out[O].st,
-                                            G.tw, nullptr, G.out[O].dst, tile, sm, O == 0, tid);
+  // non-temporal stores: the outputs are read again only by the next
+  // passes, after the whole group, so they should not evict the group's
+  // inputs, which every output re-reads
+  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, true>(G.geo, G.waves, G.phys, G.out[O].ld,
+                                                      G.out[O].st, G.tw, nullptr, G.out[O].dst,
+                                                      tile, sm, O == 0, tid);
   if constexpr (sizeof...(REST) > 0) {
     __syncthreads();  // the next pass rewrites the tile in LDS
     run_group<T, N, AX, INV, SM, O + 1>(OpList<REST...>{}, G, tile, sm);
@@ -1002,22 +1005,19 @@ using GroupSmem = Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
                        smem_tables<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
                                    group_tables<GroupOps<GRP>::AX>(typename GroupOps<GRP>::L{})>()>;
 
-// Waves per SIMD a group kernel asks the register allocator for: z groups
-// at least 3 (a budget of 168 VGPRs), x groups those of two workgroups (as
-// the single x passes run), both capped at what the LDS (160 KB per CU)
-// allows and at least one workgroup's.
+// Waves per SIMD a group kernel asks the register allocator for: those of
+// two workgroups (as the single x passes run), capped at what the LDS
+// (160 KB per CU) allows, and at least one workgroup's.
 template <typename T, int N, int GRP>
 constexpr int group_waves() {
   constexpr int AX = GroupOps<GRP>::AX;
   constexpr int w = (TileCfg<T, N, AX, 0>::BLOCK + 255) / 256;
   constexpr int lds = (int)(163840 / sizeof(GroupSmem<T, N, GRP>)) * w;
-  constexpr int want = AX == kAxZ ? (w > 3 ? w : 3) : 2 * w;
-  constexpr int r = want < lds ? want : lds;
+  constexpr int r = 2 * w < lds ? 2 * w : lds;
   return r < w ? w : (r > 8 ? 8 : r);
 }
 
-// The group runs the first pass of its transforms: the x pass of inverse
-// (x, y, z) transforms, the z pass of forward (z, y, x) ones.
+// The group runs the first (x) pass of its inverse (x, y, z) transforms.
 template <typename T, int N, int GRP>
 __global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
     __attribute__((amdgpu_waves_per_eu(group_waves<T, N, GRP>())))
