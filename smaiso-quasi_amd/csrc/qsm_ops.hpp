@@ -273,33 +273,40 @@ __device__ __forceinline__ Raw4<T> fetch(const LoadOp& L, const Geo& G, int i, i
   return r;
 }
 
+// 1/x in FP64 from the hardware reciprocal and one Newton step (x > 0)
+__device__ __forceinline__ double rcp64(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Mixed precision (T = float): the projection reads FP32 fields and writes
-// an FP32 field, so it evaluates in FP32 with the hardware reciprocal
-// (1 ulp); K^2 is a sum of squares (no cancellation).
+// an FP32 field.  Its coefficients (1/K^2 with the scale, C_M1 = scale/(nu
+// K^2), C_M2 = s_c/K^2, nu s_c) are formed in FP64 and rounded once; the
+// products with the FP32 data are FP32.
 template <int LD>
 __device__ __forceinline__ float finish32(const Raw4<float>& r, const LoadOp& L, const Phys& P,
                                           const Kv& K, const Geo& G, int i, int j, int k) {
   const Shifts h = shifts<LD>(L, G, i, j, k);
   const float a = h.fa ? r.a : 0.0f, b = h.fb ? r.b : 0.0f, c = h.fc ? r.c : 0.0f,
               dd = h.fd ? r.d : 0.0f;
-  const float nu = (float)P.nu, scale = (float)P.scale;
+  // these operators run on the x pass: the y, z terms are per thread
   if constexpr (LD == LD_PRESSURE) {
-    const float qx = (float)K.qx, qy = (float)K.qy, qz = (float)K.qz;
-    // these operators run on the x pass: the y, z terms are per thread
-    const float rm2 = __builtin_amdgcn_rcpf(qx * qx + (qy * qy + qz * qz));
-    const float pv = -scale * (qx * a + qy * b + qz * c) * rm2;
+    const float cp = (float)(-P.scale * rcp64(sq(K.qx) + (sq(K.qy) + sq(K.qz))));
+    const float pv = cp * ((float)K.qx * a + (float)K.qy * b + (float)K.qz * c);
     return (i + j + k == 0) ? 0.0f : pv;
   } else {
     // own component: wavenumber s_c; the others q
     constexpr int cc = LD == LD_VEL0 ? 0 : (LD == LD_VEL1 ? 1 : 2);
-    const float kx = (float)(cc == 0 ? K.sx : K.qx), ky = (float)(cc == 1 ? K.sy : K.qy),
-                kz = (float)(cc == 2 ? K.sz : K.qz);
-    const float rm2 = __builtin_amdgcn_rcpf(kx * kx + (ky * ky + kz * kz));
-    const float ks = cc == 0 ? kx : (cc == 1 ? ky : kz);
+    const double kx = cc == 0 ? K.sx : K.qx, ky = cc == 1 ? K.sy : K.qy, kz = cc == 2 ? K.sz : K.qz;
+    const double ks = cc == 0 ? kx : (cc == 1 ? ky : kz);
+    const double rm2 = rcp64(sq(kx) + (sq(ky) + sq(kz)));
+    const float cm1 = (float)((P.scale / P.nu) * rm2), cm2 = (float)(ks * rm2),
+                cnu = (float)(P.nu * ks);
+    const float fx = (float)kx, fy = (float)ky, fz = (float)kz;
     // read a is the own component; b, c the others in axis order
-    const float dot = cc == 0 ? kx * a + ky * b + kz * c
-                              : (cc == 1 ? kx * b + ky * a + kz * c : kx * b + ky * c + kz * a);
-    const float vv = (scale / nu) * rm2 * (a - ks * rm2 * dot + nu * ks * dd);
+    const float dot = cc == 0 ? fx * a + fy * b + fz * c
+                              : (cc == 1 ? fx * b + fy * a + fz * c : fx * b + fy * c + fz * a);
+    const float vv = cm1 * (a - cm2 * dot + cnu * dd);
     const bool in = cc == 0 ? i < G.nx - 1 : ((j < G.ny - 1) & (k < G.nz - 1));
     return in ? vv : 0.0f;
   }
