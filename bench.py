@@ -7,8 +7,10 @@ Workload (metric: grid-point updates/s at 512^3, mixed vs FP64 on one GPU):
   step), 1000 steps by default: the divergence and pressure terms switch on
   after step 200 (divv_switch), so the timed steps cover both regimes in the
   workload's proportions.  The mixed-precision path is the headline `value`;
-  the FP64 path runs beside it on the same input, and psi's relative L2
-  difference between the two at the end is reported.
+  the FP64 path runs beside it on the same input, and the relative L2
+  differences of psi, rho and u (as a vector) and the relative free-energy
+  and mass differences between the two at the end are reported
+  (`accuracy_mixed_vs_fp64`).
 
 A step is one pass of the hot path (quasi.cpp:1017-1965) over the whole grid.
 Inputs are resident in HBM before timing starts; timing is wall clock around
@@ -189,7 +191,10 @@ def main():
         if prec == qsm.MIXED:
             top = sim.timing_kernel(0)
         sim.timing(False)
-        fields[tag] = sim.field("PSI")
+        # the fields compared between the precisions (FP64 host copies; only
+        # when both runs happen, since each 512^3 copy is 1 GiB)
+        fields[tag] = {f: sim.field(f) for f in (("PSI", "RHO", "VX", "VY", "VZ") if not args.no_fp64
+                                                 else ("PSI",))}
         diag = sim.diag()
         sim.close()
         res[tag] = {"s": dt, "kt": kt, "diag": diag,
@@ -250,7 +255,16 @@ def main():
         out["fp64"] = {"value": v64, "ms_per_step": f["s"] / args.steps * 1e3, "regimes": f["regimes"],
                        "step_roofline_frac": v64 * SWEEPS_PER_STEP * 8 / 1e9 / HBM_PEAK_GBS}
         out["mixed_over_fp64"] = value / v64
-        out["psi_rel_l2_mixed_vs_fp64"] = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
+        def rel(fs):
+            num = sum(float(np.sum((a[f] - b[f]) ** 2)) for f in fs)
+            return float(np.sqrt(num / sum(float(np.sum(b[f] ** 2)) for f in fs)))
+        out["psi_rel_l2_mixed_vs_fp64"] = rel(("PSI",))
+        dm, df = res["mixed"]["diag"], f["diag"]
+        out["accuracy_mixed_vs_fp64"] = {
+            "psi_rel_l2": rel(("PSI",)), "rho_rel_l2": rel(("RHO",)),
+            "u_rel_l2": rel(("VX", "VY", "VZ")),  # velocity as a vector
+            "energy_rel": abs(dm["energy"] - df["energy"]) / abs(df["energy"]),
+            "mass_rel": abs(dm["mass"] - df["mass"]) / abs(df["mass"])}
         out["steps_compared"] = args.warmup + args.steps
     if not args.no_ensemble:
         log("ensemble")
