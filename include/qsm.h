@@ -179,9 +179,11 @@ int qsm_ens_set_graphs(qsm_ens e, int enable);
 int qsm_ens_destroy(qsm_ens e);
 
 /* ---- autotuner: times the tile-width variants of the plain transform passes
- * on the handle's (or ensemble's) own grid(s), using a scratch field, and
- * keeps the faster per axis and direction (and pencil length for ensembles).
- * cache_path (may be NULL): JSON file of earlier choices, read and updated. */
+ * on the handle's (or ensemble's) own grid(s), using a scratch field, in
+ * interleaved rounds, and keeps per axis and direction (and pencil length for
+ * ensembles) the half width only where it is at least 3% faster than the
+ * default.  cache_path (may be NULL): JSON file of earlier choices, read and
+ * updated. */
 int qsm_autotune(qsm_handle h, const char* cache_path);
 /* set the variant (0 default width, 1 half width) of the plain passes along
  * axis (0 x, 1 y, 2 z) in direction inv (1: DCT-III/DST-III) by hand */

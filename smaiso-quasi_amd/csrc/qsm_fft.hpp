@@ -292,4 +292,18 @@ QSM_PLAN(384, 12, 4, 4, 4, 3, 2)
 QSM_PLAN(512, 16, 16, 16, 2)
 #undef QSM_PLAN
 
+// Plan of a pass in precision T.  FP64 at M = 256 (N = 512) takes 8
+// elements per thread (radices 8, 8, 4): with 16 FP64 elements a strided
+// tile is one 256-thread workgroup of 8 waves per CU, with 8 twice the waves
+// (measured at 512^3: FP64 step 39.4 -> 38.1 ms; the FP32 step is faster
+// with 16).
+template <typename T, int M>
+struct PlanFor : Plan<M> {};
+template <>
+struct PlanFor<double, 256> {
+  static constexpr int E = 8;
+  using R = Radices<8, 8, 4>;
+  static constexpr int R1 = 8;
+};
+
 }  // namespace qsm

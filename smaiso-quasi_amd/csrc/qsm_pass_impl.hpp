@@ -38,7 +38,7 @@ constexpr int kAxX = 0, kAxY = 1, kAxZ = 2;
 template <typename T, int N, int AX, int V = 0>
 struct TileCfg {
   static constexpr int M = N / 2;
-  static constexpr int E = Plan<M>::E;
+  static constexpr int E = PlanFor<T, M>::E;
   static constexpr int TP = M / E;  // threads per pencil
   static constexpr int tk() {
     if (AX == kAxZ) {
@@ -277,7 +277,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
                                           const int tile, SM& sm, const bool stage = true,
                                           const int tid = (int)threadIdx.x) {
   using C = TileCfg<T, N, AX, V>;
-  using PL = Plan<N / 2>;
+  using PL = PlanFor<T, N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1, NB1 = E / R1;
   using TW = TwLayout<M>;
@@ -743,7 +743,7 @@ template <typename T, int N, int LD, class SM>
 __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int tile, SM& sm,
                                                 const int tid) {
   using C = TileCfg<T, N, kAxZ, 0>;
-  using PL = Plan<N / 2>;
+  using PL = PlanFor<T, N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1;
   constexpr int MO = kMultiGroups, R = TK / MO, NO = multi_count<LD>();
