@@ -270,6 +270,35 @@ def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec):
 
 
 @pytest.mark.parametrize("prec", (0, 1))
+def test_quasi_ensemble_matches_single_runs(qsm_mod, prec):
+    """The quasi step of a batched ensemble (one transform per launch) and of
+    each instance run alone (group x passes, the multi-output stress z pass,
+    with the divergence from step divv_switch + 1 on) agree bit for bit: 5
+    steps across the switch (divv_switch = 2), uneven grids."""
+    import ics
+    shapes = ((32, 48, 64), (64, 32, 32))
+    params, psis = [], []
+    for shape in shapes:
+        d = ics.params_dict("interface64", divv_switch=2)
+        d["nx"], d["ny"], d["nz"] = shape
+        params.append(qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_}))
+        psis.append(ics.workload_psi("interface64", shape=shape))
+    ens = qsm_mod.Ensemble(params, prec)
+    for i, psi in enumerate(psis):
+        ens.set_psi(i, psi)
+    ens.step(5)
+    for i, psi in enumerate(psis):
+        sim = qsm_mod.Sim(params[i], prec)
+        sim.set_psi(psi)
+        sim.step(5)
+        for f in ("PSI", "PSIHAT", "RHO", "VX", "VY", "VZ", "P", "NHAT", "DHAT", "SX", "SZ"):
+            a, b = ens.field(i, f), sim.field(f)
+            print(f"instance {i} {f}: rel-L2 {rel_l2(a, b):.2e}", flush=True)
+            assert np.array_equal(a, b), (i, f)
+        sim.close()
+
+
+@pytest.mark.parametrize("prec", (0, 1))
 def test_tile_variants_agree(qsm_mod, prec):
     """The half-width tile variant of the plain passes (chosen by the
     autotuner) regroups pencils into workgroups without changing the
