@@ -48,7 +48,13 @@ struct TileCfg {
       while (t > 1 && (long)M * (t + 1) * 2 * (long)sizeof(T) > 72000) t /= 2;
       return t;
     }
-    // strided axes: 128 contiguous bytes of each row per tile
+    // strided axes: 128 contiguous bytes of each row per tile.  FP64 x
+    // passes take 256 (32 columns) where the tile fits the LDS (M <= 256,
+    // one workgroup per CU): the 1-2 MiB row stride of the x pass reaches
+    // 5.2 TB/s with 256-byte row segments against 4.5 TB/s with 128 (copy
+    // microbenchmark), and the FP64 step went from 38.2 to 36.0 ms at 512^3
+    // (the same for FP64 y passes, or for FP32, is slower)
+    if (AX == kAxX && sizeof(T) == 8 && (long)M * 33 * 16 <= 140000) return 32;
     int t = 128 / (int)sizeof(T);
     while (t > 1 && (long)M * (t + 1) * 2 * (long)sizeof(T) > 72000) t /= 2;
     return t;
