@@ -48,15 +48,17 @@ struct TileCfg {
       while (t > 1 && (long)M * (t + 1) * 2 * (long)sizeof(T) > 72000) t /= 2;
       return t;
     }
-    // strided axes: 128 contiguous bytes of each row per tile.  FP64 x
-    // passes take 256 (32 columns) where the tile fits the LDS (M <= 256,
-    // one workgroup per CU): the 1-2 MiB row stride of the x pass reaches
-    // 5.2 TB/s with 256-byte row segments against 4.5 TB/s with 128 (copy
-    // microbenchmark), and the FP64 step went from 38.2 to 36.0 ms at 512^3
-    // (the same for FP64 y passes, or for FP32, is slower)
-    if (AX == kAxX && sizeof(T) == 8 && (long)M * 33 * 16 <= 140000) return 32;
-    int t = 128 / (int)sizeof(T);
-    while (t > 1 && (long)M * (t + 1) * 2 * (long)sizeof(T) > 72000) t /= 2;
+    // strided axes: 128 contiguous bytes of each row per tile (FP64 x
+    // passes: 256), narrowed only where the tile does not fit one workgroup
+    // per CU (tile slots (t+1)*M plus the 3M twiddle table within the
+    // CU's 160 KiB of LDS).  Wider row segments carry the strided
+    // access pattern (copy microbenchmarks: x stride 4.2 / 5.1 / 5.2 TB/s
+    // with 64 / 128 / 256-byte segments in FP64 and FP32); at 512^3 the
+    // 128-byte tiles also fit two workgroups per CU.  FP64 x at 256 bytes:
+    // FP64 step 38.2 -> 36.0 ms at 512^3 (FP64 y and FP32 at 256 are slower).
+    const int want = (AX == kAxX && sizeof(T) == 8) ? 32 : 128 / (int)sizeof(T);
+    int t = want;
+    while (t > 1 && (long)M * (t + 4) * 2 * (long)sizeof(T) > 160 * 1024 - 256) t /= 2;
     return t;
   }
   static constexpr int TK = (V == 1 && tk() > 1) ? tk() / 2 : tk();
