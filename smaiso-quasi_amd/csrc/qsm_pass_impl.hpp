@@ -259,8 +259,10 @@ struct Smem {
   cx<T> ts[M];         // stage twiddles
   cx<T> tp[NTP];       // pre- (inverse) or post-processing (forward) twiddles
   // the tables (ntab * N values), then the per-row values (nrowt); one array,
-  // so a pass without tables adds 8 bytes, not 16
-  double xt[(ntab * N + nrowt) > 0 ? ntab * N + nrowt : 1];
+  // of length 0 in a pass without tables: a placeholder element would take
+  // the FP64 inverse y tile at 512 (81920 bytes) past half the CU's LDS, to
+  // one workgroup per CU
+  double xt[ntab * N + nrowt];
 };
 
 // Tables staged in LDS: those of mask NT where they fit beside the tile while
