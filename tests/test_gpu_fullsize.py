@@ -1,0 +1,180 @@
+"""GPU checks at the full workload sizes: 512^3 (the benchmark grid) and
+1024x1024x512 (2^29 points, the largest field the library accepts).
+
+The oracle does not finish these sizes in seconds, so the checks use
+properties fixed by the transform definitions and by the step itself:
+
+* single mode: FFTW's REDFT10 / RODFT10 (quasi.cpp:449-495 plans them) map
+  the separable basis function cos(pi m (n + 1/2) / N) resp.
+  sin(pi (m + 1) (n + 1/2) / N) per axis to one coefficient N at index m
+  (discrete orthogonality, 0 < m < N - 1), so the 3D output is
+  Nx Ny Nz at (mx, my, mz) and zero elsewhere.  A wrong index, a swapped
+  axis, a DST sign or a lost factor moves or rescales that peak;
+* inverse mode: REDFT01 / RODFT01 of a unit coefficient at (mx, my, mz)
+  is the product of 2 cos(pi m (k + 1/2) / N) resp.
+  2 sin(pi (m + 1) (k + 1/2) / N) over the axes;
+* round trip: forward then inverse times 1 / (8 Nx Ny Nz) is the identity
+  (the reference's `scale`, quasi.cpp:219);
+* linearity of the 512^3 FP64 transform on two random fields;
+* the step: mixed precision against FP64 on the same initial condition,
+  the precision contract of the mixed path (psi rel-L2 <= 1e-5).
+
+Tolerances: FP64 1e-12 (peak-relative max error, or rel-L2), FP32 1e-6
+peak-relative and 2e-6 rel-L2, the FP32 transform tolerance of the
+small-size parity tests.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SIZES = ((512, 512, 512), (1024, 1024, 512))
+MODES = {(512, 512, 512): (37, 5, 131), (1024, 1024, 512): (7, 300, 129)}
+
+
+def _fwd_basis(kind, n, m):
+    j = np.arange(n, dtype=np.float64) + 0.5
+    if kind == 5:  # REDFT10
+        return np.cos(np.pi * m * j / n)
+    assert kind == 9  # RODFT10
+    return np.sin(np.pi * (m + 1) * j / n)
+
+
+def _inv_basis(kind, n, m):
+    k = np.arange(n, dtype=np.float64) + 0.5
+    if kind == 4:  # REDFT01
+        return 2.0 * np.cos(np.pi * m * k / n)
+    assert kind == 8  # RODFT01
+    return 2.0 * np.sin(np.pi * (m + 1) * k / n)
+
+
+def _outer(a, b, c, dtype):
+    ab = (a[:, None] * b[None, :]).astype(dtype)
+    return ab[:, :, None] * c.astype(dtype)[None, None, :]
+
+
+def _prec_tol(qsm, prec):
+    return (1e-12, 1e-12) if prec == qsm.FP64 else (1e-6, 2e-6)
+
+
+@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("prec", (0, 1))
+def test_single_mode_forward(qsm_mod, shape, prec):
+    prec = qsm_mod.FP64 if prec == 0 else qsm_mod.MIXED
+    dtype = np.float64 if prec == qsm_mod.FP64 else np.float32
+    peak_tol, _ = _prec_tol(qsm_mod, prec)
+    mx, my, mz = MODES[shape]
+    for kinds in ([5, 9, 5], [9, 5, 9]):
+        print(f"single mode fwd shape={shape} prec={prec} kinds={kinds}", flush=True)
+        x = _outer(*(_fwd_basis(kd, n, m) for kd, n, m in zip(kinds, shape, (mx, my, mz))),
+                   dtype)
+        plan = qsm_mod.Plan(shape, kinds, prec)
+        y = plan(x)
+        plan.close()
+        del x
+        peak = float(shape[0]) * shape[1] * shape[2]
+        assert np.unravel_index(int(np.argmax(np.abs(y))), shape) == (mx, my, mz)
+        got = float(y[mx, my, mz])
+        y[mx, my, mz] = 0
+        off = float(np.max(np.abs(y)))
+        print(f"  peak {got / peak - 1:.2e} off-peak {off / peak:.2e}", flush=True)
+        assert abs(got / peak - 1.0) < peak_tol
+        assert off / peak < peak_tol
+
+
+@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("prec", (0, 1))
+def test_single_mode_inverse(qsm_mod, shape, prec):
+    prec = qsm_mod.FP64 if prec == 0 else qsm_mod.MIXED
+    dtype = np.float64 if prec == qsm_mod.FP64 else np.float32
+    _, l2_tol = _prec_tol(qsm_mod, prec)
+    mx, my, mz = MODES[shape]
+    for kinds in ([4, 8, 4], [8, 4, 8]):
+        print(f"single mode inv shape={shape} prec={prec} kinds={kinds}", flush=True)
+        x = np.zeros(shape, dtype=dtype)
+        x[mx, my, mz] = 1.0
+        plan = qsm_mod.Plan(shape, kinds, prec)
+        y = plan(x)
+        plan.close()
+        del x
+        want = _outer(*(_inv_basis(kd, n, m) for kd, n, m in zip(kinds, shape, (mx, my, mz))),
+                      dtype)
+        e = rel_l2(y, want)
+        print(f"  rel_l2 {e:.2e}", flush=True)
+        assert e < l2_tol
+
+
+@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("prec", (0, 1))
+def test_round_trip(qsm_mod, shape, prec):
+    prec = qsm_mod.FP64 if prec == 0 else qsm_mod.MIXED
+    dtype = np.float64 if prec == qsm_mod.FP64 else np.float32
+    _, l2_tol = _prec_tol(qsm_mod, prec)
+    print(f"round trip shape={shape} prec={prec}", flush=True)
+    x = np.random.default_rng(11).standard_normal(shape, dtype=dtype)
+    f = qsm_mod.Plan(shape, [5, 9, 5], prec)
+    b = qsm_mod.Plan(shape, [4, 8, 4], prec)
+    y = b(f(x))
+    f.close()
+    b.close()
+    y *= dtype(1.0 / (8.0 * shape[0] * shape[1] * shape[2]))
+    e = rel_l2(y, x)
+    print(f"  rel_l2 {e:.2e}", flush=True)
+    assert e < (1e-13 if prec == qsm_mod.FP64 else l2_tol)
+
+
+def test_linearity_512_fp64(qsm_mod):
+    shape = SIZES[0]
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(shape)
+    z = rng.standard_normal(shape)
+    plan = qsm_mod.Plan(shape, [9, 5, 9], qsm_mod.FP64)
+    tx, tz = plan(x), plan(z)
+    x *= 0.75
+    x -= 1.5 * z
+    del z
+    t = plan(x)
+    plan.close()
+    tx *= 0.75
+    tx -= 1.5 * tz
+    e = rel_l2(t, tx)
+    print(f"linearity 512^3 fp64 rel_l2 {e:.2e}", flush=True)
+    assert e < 1e-13
+
+
+def _run(qsm, name, prec, steps, psi):
+    import ics
+    d = ics.params_dict(name)
+    sim = qsm.Sim(qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_}), prec)
+    sim.set_psi(psi)
+    sim.step(steps)
+    out = {f: sim.field(f) for f in ("PSI", "RHO")}
+    dg = sim.diag()
+    sim.close()
+    return out, dg
+
+
+@pytest.mark.parametrize("name,steps", (("tilted512", 100), ("sphere1024", 20)))
+def test_step_mixed_matches_fp64(qsm_mod, name, steps):
+    """The mixed path against FP64 on the workloads the benchmark and the
+    largest configuration run (the bench reports the 1000-step figures)."""
+    import ics
+    print(f"step {name} {steps} steps: initial condition", flush=True)
+    psi = ics.workload_psi(name)
+    print("  fp64", flush=True)
+    a, da = _run(qsm_mod, name, qsm_mod.FP64, steps, psi)
+    print("  mixed", flush=True)
+    b, db = _run(qsm_mod, name, qsm_mod.MIXED, steps, psi)
+    del psi
+    for f, tol in (("PSI", 1e-5), ("RHO", 1e-6)):
+        assert np.all(np.isfinite(b[f])), f
+        e = rel_l2(b[f], a[f])
+        print(f"  {f}: rel_l2 mixed vs fp64 {e:.2e} (tol {tol:.0e})", flush=True)
+        assert e < tol, (name, f, e)
+    for k in ("energy", "mass"):
+        va, vb = da[k], db[k]
+        r = abs(vb - va) / abs(va)
+        print(f"  {k}: fp64 {va:.10e} mixed rel {r:.2e}", flush=True)
+        assert r < 1e-5, (name, k, r)
