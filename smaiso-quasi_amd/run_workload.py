@@ -8,7 +8,9 @@
 --compare runs the FP64 path beside the mixed one and reports the relative L2
 difference of psi, rho and v (as a vector) and the relative energy and mass
 differences every --every steps; --compare-at K compares at step K only (the
-FP64 handle is then freed and the mixed run continues alone).  Output: one
+FP64 handle is then freed and the mixed run continues alone).  --divv-switch
+S overrides the workload's divv_switch (the step after which D is computed,
+quasi.cpp:1014-1965), so a short run reaches the later regime.  Output: one
 JSON object per report line on stdout.
 """
 import argparse
@@ -52,10 +54,12 @@ def main():
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--compare", action="store_true")
     ap.add_argument("--compare-at", type=int, default=0)
+    ap.add_argument("--divv-switch", type=int, default=None)
     args = ap.parse_args()
     w = ics.WORKLOADS[args.workload]
     steps = args.steps or w["steps"]
-    d = ics.params_dict(args.workload)
+    over = {} if args.divv_switch is None else {"divv_switch": args.divv_switch}
+    d = ics.params_dict(args.workload, **over)
     p = qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_})
     t0 = time.time()
     psi = ics.workload_psi(args.workload)
