@@ -937,7 +937,8 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
 }
 
 // ---- group passes (the x pass of inverse transforms): the operators of
-// each group in output order
+// each group in output order, and the workgroups that share each tile in
+// FP32 (PARTS; times per launch at 512^3)
 template <int... LDs>
 struct OpList {};
 template <int G>
@@ -949,6 +950,7 @@ struct GroupOps<GRP_PSI> {
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
+  static constexpr int PARTS = 2;  // 1495 -> 1420 us (4 and 8 parts: slower)
 };
 template <>
 struct GroupOps<GRP_RHO> {
@@ -956,6 +958,7 @@ struct GroupOps<GRP_RHO> {
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
+  static constexpr int PARTS = 1;  // 3 parts: 656 -> 710 us
 };
 template <>
 struct GroupOps<GRP_PROJ> {
@@ -963,6 +966,7 @@ struct GroupOps<GRP_PROJ> {
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = true;
+  static constexpr int PARTS = 1;  // 2 parts: 1478 -> 1628 us
 };
 template <>
 struct GroupOps<GRP_STB_PSI> {
@@ -970,6 +974,7 @@ struct GroupOps<GRP_STB_PSI> {
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
+  static constexpr int PARTS = 1;
 };
 template <>
 struct GroupOps<GRP_STB_PROJ> {
@@ -977,6 +982,7 @@ struct GroupOps<GRP_STB_PROJ> {
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = true;
+  static constexpr int PARTS = 1;
 };
 
 template <int AX, int... LDs>
@@ -988,25 +994,28 @@ constexpr int group_count(OpList<LDs...>) {
   return (int)sizeof...(LDs);
 }
 
-// Output O of a group: its pass on this tile, then the next output's.  The
-// passes index the same tile, so the compiler would keep every per-element
-// address of the first pass live for reuse in the next ones; an opaque copy
-// of the thread index per pass makes each pass derive its own, so each keeps
-// the register budget of a single pass.
-template <typename T, int N, int AX, bool INV, class SM, int O, int LD, int... REST>
+// Outputs LO..HI-1 of a group: output O's pass on this tile, then the next
+// one's.  The passes index the same tile, so the compiler would keep every
+// per-element address of the first pass live for reuse in the next ones; an
+// opaque copy of the thread index per pass makes each pass derive its own,
+// so each keeps the register budget of a single pass.
+template <typename T, int N, int AX, bool INV, class SM, int O, int LO, int HI, int LD,
+          int... REST>
 __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& G, int tile,
                                           SM& sm) {
-  int tid = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  // non-temporal stores: the outputs are read again only by the next
-  // passes, after the whole group, so they should not evict the group's
-  // inputs, which every output re-reads
-  pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, true>(G.geo, G.waves, G.phys, G.out[O].ld,
-                                                      G.out[O].st, G.tw, nullptr, G.out[O].dst,
-                                                      tile, sm, O == 0, tid);
-  if constexpr (sizeof...(REST) > 0) {
-    __syncthreads();  // the next pass rewrites the tile in LDS
-    run_group<T, N, AX, INV, SM, O + 1>(OpList<REST...>{}, G, tile, sm);
+  if constexpr (O >= LO && O < HI) {
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    // non-temporal stores: the outputs are read again only by the next
+    // passes, after the whole group, so they should not evict the group's
+    // inputs, which every output re-reads
+    pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, true>(G.geo, G.waves, G.phys, G.out[O].ld,
+                                                        G.out[O].st, G.tw, nullptr, G.out[O].dst,
+                                                        tile, sm, O == LO, tid);
+  }
+  if constexpr (sizeof...(REST) > 0 && O + 1 < HI) {
+    if constexpr (O >= LO) __syncthreads();  // the next pass rewrites the tile in LDS
+    run_group<T, N, AX, INV, SM, O + 1, LO, HI>(OpList<REST...>{}, G, tile, sm);
   }
 }
 
@@ -1027,19 +1036,57 @@ constexpr int group_waves() {
   return r < w ? w : (r > 8 ? 8 : r);
 }
 
+// Parts per tile of group GRP in precision T (FP32 only: the FP64 group
+// tiles take one workgroup per CU, and splitting them measured no gain).
+template <typename T, int GRP>
+constexpr int group_parts() {
+  return sizeof(T) == 4 ? GroupOps<GRP>::PARTS : 1;
+}
+
+// Part P of PARTS of a group's outputs on this tile: outputs
+// [P CNT / PARTS, (P + 1) CNT / PARTS).
+template <typename T, int N, int AX, bool INV, class SM, class L, int CNT, int PARTS, int P>
+__device__ __forceinline__ void run_part(int part, const GroupArgs& G, int tile, SM& sm) {
+  if constexpr (P < PARTS) {
+    if (part == P)
+      run_group<T, N, AX, INV, SM, 0, P * CNT / PARTS, (P + 1) * CNT / PARTS>(L{}, G, tile, sm);
+    else
+      run_part<T, N, AX, INV, SM, L, CNT, PARTS, P + 1>(part, G, tile, sm);
+  }
+}
+
 // The group runs the first (x) pass of its inverse (x, y, z) transforms.
+// A group with PARTS > 1 runs each tile as PARTS workgroups, each on its
+// share of the outputs: workgroups b, b + 8, b + 16, ... (one XCD,
+// dispatched one after the other) take the parts of one tile, so they read
+// its inputs at about the same time through one L2, and each re-reads them
+// for fewer outputs.
 template <typename T, int N, int GRP>
 __global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
     __attribute__((amdgpu_waves_per_eu(group_waves<T, N, GRP>())))
     k_group(const GroupArgs G) {
   using GO = GroupOps<GRP>;
   constexpr int AX = GO::AX;
+  constexpr int CNT = group_count(typename GO::L{});
+  constexpr int PARTS = group_parts<T, GRP>();
+  static_assert(PARTS >= 1 && PARTS <= CNT, "every part runs at least one output");
   __shared__ GroupSmem<T, N, GRP> sm;
+  int b = (int)blockIdx.x, nb = (int)gridDim.x, part = 0;
+  if constexpr (PARTS > 1) {
+    nb /= PARTS;
+    if (nb % 8 == 0) {
+      const int x = b % 8, s = b / 8;
+      part = s % PARTS;
+      b = (s / PARTS) * 8 + x;
+    } else {
+      part = b % PARTS;
+      b /= PARTS;
+    }
+  }
   // projection groups read the j-1 and j+1 neighbours of their rows: one
   // contiguous range of rows per XCD keeps those in the XCD's L2
-  const int tile = GO::XORD ? xcd_tile((int)blockIdx.x, (int)gridDim.x)
-                            : tile_order<T, N, AX, 0>(G.geo, (int)blockIdx.x, (int)gridDim.x);
-  run_group<T, N, AX, GO::INV, decltype(sm), 0>(typename GO::L{}, G, tile, sm);
+  const int tile = GO::XORD ? xcd_tile(b, nb) : tile_order<T, N, AX, 0>(G.geo, b, nb);
+  run_part<T, N, AX, GO::INV, decltype(sm), typename GO::L, CNT, PARTS, 0>(part, G, tile, sm);
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
@@ -1075,7 +1122,7 @@ hipError_t launch_group_g(const GroupArgs& G, long* nblocks, hipStream_t s) {
     return hipErrorInvalidValue;
   } else {
     using C = TileCfg<T, N, AX, 0>;
-    const long nb = tiles_of<T, N, AX, 0>(G.geo);
+    const long nb = tiles_of<T, N, AX, 0>(G.geo) * group_parts<T, GG>();
     if (nblocks) *nblocks = nb;
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_group<T, N, GG>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, G);
