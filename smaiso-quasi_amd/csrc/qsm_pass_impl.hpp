@@ -248,6 +248,7 @@ template <typename T, int N, int AX, bool INV, int V, int TAB>
 struct Smem {
   using C = TileCfg<T, N, AX, V>;
   static constexpr int M = C::M, TK = C::TK, TKP = C::TKP;
+  static constexpr int VAR = V;
   static constexpr int NTP = INV ? TwLayout<M>::NPRE : TwLayout<M>::NPOST;
   static constexpr int tab = TAB, ntab = kv_count(TAB);
   static constexpr int nrowt = (AX == 2 && ntab > 0) ? 4 * TK : 0;
@@ -937,8 +938,10 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
 }
 
 // ---- group passes (the x pass of inverse transforms): the operators of
-// each group in output order, and the workgroups that share each tile in
-// FP32 (PARTS; times per launch at 512^3)
+// each group in output order, the workgroups that share each tile in FP32
+// (PARTS) and the FP64 tile variant (VAR64); times per launch at 512^3.
+// FP64 psi and rho groups at half width: 3765 -> 3370 and 1630 -> 1530 us
+// (the projection group: no gain; FP32 groups at half width: slower)
 template <int... LDs>
 struct OpList {};
 template <int G>
@@ -951,6 +954,7 @@ struct GroupOps<GRP_PSI> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
   static constexpr int PARTS = 2;  // 1495 -> 1420 us (4 and 8 parts: slower)
+  static constexpr int VAR64 = 1;
 };
 template <>
 struct GroupOps<GRP_RHO> {
@@ -959,6 +963,7 @@ struct GroupOps<GRP_RHO> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
   static constexpr int PARTS = 1;  // 3 parts: 656 -> 710 us
+  static constexpr int VAR64 = 1;
 };
 template <>
 struct GroupOps<GRP_PROJ> {
@@ -967,6 +972,7 @@ struct GroupOps<GRP_PROJ> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = true;
   static constexpr int PARTS = 1;  // 2 parts: 1478 -> 1628 us
+  static constexpr int VAR64 = 0;
 };
 template <>
 struct GroupOps<GRP_STB_PSI> {
@@ -975,6 +981,7 @@ struct GroupOps<GRP_STB_PSI> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
   static constexpr int PARTS = 1;
+  static constexpr int VAR64 = 0;
 };
 template <>
 struct GroupOps<GRP_STB_PROJ> {
@@ -983,6 +990,7 @@ struct GroupOps<GRP_STB_PROJ> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = true;
   static constexpr int PARTS = 1;
+  static constexpr int VAR64 = 0;
 };
 
 template <int AX, int... LDs>
@@ -1009,7 +1017,7 @@ __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& 
     // non-temporal stores: the outputs are read again only by the next
     // passes, after the whole group, so they should not evict the group's
     // inputs, which every output re-reads
-    pass_tile<T, N, AX, INV, LD, ST_PLAIN, 0, SM, true>(G.geo, G.waves, G.phys, G.out[O].ld,
+    pass_tile<T, N, AX, INV, LD, ST_PLAIN, SM::VAR, SM, true>(G.geo, G.waves, G.phys, G.out[O].ld,
                                                         G.out[O].st, G.tw, nullptr, G.out[O].dst,
                                                         tile, sm, O == LO, tid);
   }
@@ -1019,10 +1027,19 @@ __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& 
   }
 }
 
+// Tile variant of group GRP in precision T: FP64 groups with VAR64 = 1 take
+// half-width tiles (128-byte row segments, two workgroups per CU instead of
+// one); FP32 groups keep full width.
+template <typename T, int GRP>
+constexpr int group_variant() {
+  return sizeof(T) == 8 ? GroupOps<GRP>::VAR64 : 0;
+}
+
 template <typename T, int N, int GRP>
-using GroupSmem = Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
-                       smem_tables<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
-                                   group_tables<GroupOps<GRP>::AX>(typename GroupOps<GRP>::L{})>()>;
+using GroupSmem =
+    Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, group_variant<T, GRP>(),
+         smem_tables<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, group_variant<T, GRP>(),
+                     group_tables<GroupOps<GRP>::AX>(typename GroupOps<GRP>::L{})>()>;
 
 // Waves per SIMD a group kernel asks the register allocator for: those of
 // two workgroups (as the single x passes run), capped at what the LDS
@@ -1030,14 +1047,14 @@ using GroupSmem = Smem<T, N, GroupOps<GRP>::AX, GroupOps<GRP>::INV, 0,
 template <typename T, int N, int GRP>
 constexpr int group_waves() {
   constexpr int AX = GroupOps<GRP>::AX;
-  constexpr int w = (TileCfg<T, N, AX, 0>::BLOCK + 255) / 256;
+  constexpr int w = (TileCfg<T, N, AX, group_variant<T, GRP>()>::BLOCK + 255) / 256;
   constexpr int lds = (int)(163840 / sizeof(GroupSmem<T, N, GRP>)) * w;
   constexpr int r = 2 * w < lds ? 2 * w : lds;
   return r < w ? w : (r > 8 ? 8 : r);
 }
 
-// Parts per tile of group GRP in precision T (FP32 only: the FP64 group
-// tiles take one workgroup per CU, and splitting them measured no gain).
+// Parts per tile of group GRP in precision T (FP32 only: splitting the FP64
+// psi group, at either tile width, measured no gain).
 template <typename T, int GRP>
 constexpr int group_parts() {
   return sizeof(T) == 4 ? GroupOps<GRP>::PARTS : 1;
@@ -1062,7 +1079,7 @@ __device__ __forceinline__ void run_part(int part, const GroupArgs& G, int tile,
 // its inputs at about the same time through one L2, and each re-reads them
 // for fewer outputs.
 template <typename T, int N, int GRP>
-__global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
+__global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, group_variant<T, GRP>()>::BLOCK))
     __attribute__((amdgpu_waves_per_eu(group_waves<T, N, GRP>())))
     k_group(const GroupArgs G) {
   using GO = GroupOps<GRP>;
@@ -1085,7 +1102,8 @@ __global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, 0>::BLOCK))
   }
   // projection groups read the j-1 and j+1 neighbours of their rows: one
   // contiguous range of rows per XCD keeps those in the XCD's L2
-  const int tile = GO::XORD ? xcd_tile(b, nb) : tile_order<T, N, AX, 0>(G.geo, b, nb);
+  const int tile = GO::XORD ? xcd_tile(b, nb)
+                            : tile_order<T, N, AX, group_variant<T, GRP>()>(G.geo, b, nb);
   run_part<T, N, AX, GO::INV, decltype(sm), typename GO::L, CNT, PARTS, 0>(part, G, tile, sm);
 }
 
@@ -1121,8 +1139,9 @@ hipError_t launch_group_g(const GroupArgs& G, long* nblocks, hipStream_t s) {
   if constexpr (GroupOps<GG>::AX != AX) {
     return hipErrorInvalidValue;
   } else {
-    using C = TileCfg<T, N, AX, 0>;
-    const long nb = tiles_of<T, N, AX, 0>(G.geo) * group_parts<T, GG>();
+    constexpr int GV = group_variant<T, GG>();
+    using C = TileCfg<T, N, AX, GV>;
+    const long nb = tiles_of<T, N, AX, GV>(G.geo) * group_parts<T, GG>();
     if (nblocks) *nblocks = nb;
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_group<T, N, GG>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, G);
