@@ -206,6 +206,12 @@ int qsm_timing_get(qsm_handle h, int which, double* total_ms, long long* launche
  * bytes */
 int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, int* ld, int* st, int* variant,
                       double* total_ms, long long* launches, double* alg_bytes);
+/* reset the counters and time only the launches of one kernel, named by the
+ * fields qsm_timing_kernel reports (axis, inv, ld, st, variant), so that a
+ * measured run records two events per launch of that kernel instead of two
+ * per launch of every kernel (about 3 us of device time per event pair);
+ * qsm_timing_enable clears the selection */
+int qsm_timing_select(qsm_handle h, int axis, int inv, int ld, int st, int variant);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,8 @@ bool kind_dst(int k) { return k == QSM_RODFT01 || k == QSM_RODFT10; }
 // k_pass<T, N, axis, inv, LD, ST, V> (key = (((axis*2 + inv)*64 + LD)*16 + ST)*2 + V).
 struct Timer {
   bool on = false;
+  int only = -1;  // >= 0: record only the launches of this kernel key
+  bool want(int key) const { return on && (only < 0 || only == key); }
   struct Rec { hipEvent_t a, b; int cat; double bytes; };
   struct Acc { double ms = 0, bytes = 0; long long count = 0; };
   std::vector<Rec> recs;
@@ -379,13 +381,15 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
     return QSM_OK;
   }
   Timer& T = h->timer;
+  const int key = timer_key(axis, inv, L.code, S.code, var);
+  const bool timed = T.want(key);
   hipEvent_t ea = nullptr, eb = nullptr;
-  if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+  if (timed) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
   hipError_t e = launch_pass(h->fp32, n, axis, inv, L.code, S.code, A, nullptr, h->stream, nullptr, var);
   if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch (axis %d, n %d): %s", axis, n, hipGetErrorString(e));
-  if (T.on) {
+  if (timed) {
     hipEventRecord(eb, h->stream);
-    T.recs.push_back({ea, eb, timer_key(axis, inv, L.code, S.code, var), op_bytes(h, L, S, first_pass)});
+    T.recs.push_back({ea, eb, key, op_bytes(h, L, S, first_pass)});
   }
   return QSM_OK;
 }
@@ -465,14 +469,15 @@ int run_group_first(qsm_handle_s* h, int group, const GroupItem* it, int count, 
   if (h->rec) return fail(QSM_ERR_STATE, "group passes cannot be recorded");
   const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
   Timer& T = h->timer;
+  const int key = timer_key(axis, group_inverse(group), kGroupCode0 + group, ST_PLAIN, 0);
+  const bool timed = T.want(key);
   hipEvent_t ea = nullptr, eb = nullptr;
-  if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+  if (timed) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
   hipError_t e = launch_group(h->fp32, n, group, G, nullptr, h->stream);
   if (e != hipSuccess) return fail(QSM_ERR_HIP, "group %d launch: %s", group, hipGetErrorString(e));
-  if (T.on) {
+  if (timed) {
     hipEventRecord(eb, h->stream);
-    T.recs.push_back({ea, eb, timer_key(axis, group_inverse(group), kGroupCode0 + group, ST_PLAIN, 0),
-                      (double)(reads + count) * (double)h->n * (double)h->esz});
+    T.recs.push_back({ea, eb, key, (double)(reads + count) * (double)h->n * (double)h->esz});
   }
   return QSM_OK;
 }
@@ -509,14 +514,15 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int
     A.dmask = 1 << 2;  // component z: DST along z
     for (int c = 0; c < nout; ++c) A.out[c] = out[c];
     Timer& T = h->timer;
+    const int key = timer_key(2, false, L.code, ST_PLAIN, 0);
+    const bool timed = T.want(key);
     hipEvent_t ea = nullptr, eb = nullptr;
-    if (T.on) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
+    if (timed) { ea = T.get(); eb = T.get(); hipEventRecord(ea, h->stream); }
     hipError_t e = launch_multi(h->fp32, h->p.nz, A, nullptr, h->stream);
     if (e != hipSuccess) return fail(QSM_ERR_HIP, "multi-output launch: %s", hipGetErrorString(e));
-    if (T.on) {
+    if (timed) {
       hipEventRecord(eb, h->stream);
-      T.recs.push_back({ea, eb, timer_key(2, false, L.code, ST_PLAIN, 0),
-                        (double)(reads + nout) * (double)h->n * (double)h->esz});
+      T.recs.push_back({ea, eb, key, (double)(reads + nout) * (double)h->n * (double)h->esz});
     }
   }
   for (int c = 0; c < nout; ++c) {
@@ -1133,6 +1139,18 @@ extern "C" int qsm_timing_enable(qsm_handle h, int enable) {
   if (!h) return fail(QSM_ERR_ARG, "null handle");
   h->timer.reset();
   h->timer.on = enable != 0;
+  h->timer.only = -1;
+  return QSM_OK;
+}
+
+extern "C" int qsm_timing_select(qsm_handle h, int axis, int inv, int ld, int st, int variant) {
+  if (!h) return fail(QSM_ERR_ARG, "null handle");
+  if (axis < 0 || axis > 2 || ld < 0 || ld >= 64 || st < 0 || st >= 16 || variant < 0 || variant > 1)
+    return fail(QSM_ERR_ARG, "qsm_timing_select: bad kernel (axis %d ld %d st %d variant %d)",
+                axis, ld, st, variant);
+  h->timer.reset();
+  h->timer.on = true;
+  h->timer.only = timer_key(axis, inv != 0, ld, st, variant);
   return QSM_OK;
 }
 

@@ -95,6 +95,7 @@ SIGNATURES = {
     "qsm_ens_autotune": ([_VP, ctypes.c_char_p], _IP),
     "qsm_set_variant": ([_VP, _IP, _IP, _IP], _IP),
     "qsm_timing_enable": ([_VP, _IP], _IP),
+    "qsm_timing_select": ([_VP] + [_IP] * 5, _IP),
     "qsm_timing_get": ([_VP, _IP, _DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
     "qsm_timing_kernel": ([_VP, _IP] + [ctypes.POINTER(_IP)] * 5 +
                           [_DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
@@ -232,6 +233,10 @@ class Sim:
 
     def timing(self, enable=True):
         _check(lib().qsm_timing_enable(self.h, 1 if enable else 0))
+
+    def timing_select(self, axis, inv, ld, st, variant=0):
+        """Time only the launches of one kernel (the fields of timing_kernel)."""
+        _check(lib().qsm_timing_select(self.h, axis, inv, ld, st, variant))
 
     def timing_get(self, which=0):
         ms = ctypes.c_double()

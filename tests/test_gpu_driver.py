@@ -68,3 +68,40 @@ def test_driver_reference_outputs(qsm_mod, tmp_path):
     sim.checkpoint_read(str(tmp_path / "end.ckpt"))
     assert sim.diag()["step"] == 20
     assert abs(sim.diag()["L1"] - l1[-1]) <= 1e-5 * abs(l1[-1])
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_timing_select_records_one_kernel(qsm_mod, prec):
+    """qsm_timing_select: the counters hold that kernel's launches only, as
+    many per step as full timing counts for it, and the step's results do
+    not depend on which launches are timed."""
+    import ics
+    d = ics.params_dict("interface64")
+    d["nx"], d["ny"], d["nz"] = (32, 32, 64)
+    qp = qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_})
+    psi = ics.workload_psi("interface64", shape=(32, 32, 64))
+    a = qsm_mod.Sim(qp, prec)
+    a.set_psi(psi)
+    a.step(2)
+    a.timing(True)
+    a.step(3)
+    full = a.timing_kernel(0)
+    second = a.timing_kernel(1)
+    a.timing_select(full["axis"], full["inv"], full["ld"], full["st"], full["variant"])
+    a.step(3)
+    sel = a.timing_kernel(0)
+    for k in ("axis", "inv", "ld", "st", "variant", "launches"):
+        assert sel[k] == full[k], (k, sel, full)
+    assert sel["bytes"] == full["bytes"]
+    assert 0 < sel["ms"]
+    with pytest.raises(qsm_mod.QsmError):
+        a.timing_kernel(1)  # nothing else was timed
+    assert second["launches"] > 0
+    a.timing(False)
+    b = qsm_mod.Sim(qp, prec)
+    b.set_psi(psi)
+    b.step(8)
+    for f in ("PSI", "RHO"):
+        assert np.array_equal(a.field(f), b.field(f)), f
+    with pytest.raises(qsm_mod.QsmError):
+        a.timing_select(3, 0, 0, 0, 0)
