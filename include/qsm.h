@@ -148,6 +148,11 @@ int qsm_set_graphs(qsm_handle h, int enable);
 /* run until L1 <= lim_l1 (checked every step_l1 steps, quasi.cpp:1014, 1972,
  * 2010) or max_steps; *steps_done receives the steps taken */
 int qsm_run_l1(qsm_handle h, int max_steps, int step_l1, double lim_l1, int* steps_done);
+/* copy field `which` of the current step to host (caller-owned, nx*ny*nz
+ * doubles).  Fields no later stage of the step reads are formed here on
+ * request: D (QSM_DIVV) always, p (QSM_P) up to step divv_switch + 1.  The
+ * lagged grad rho (QSM_RX..RZ) exists at step 1 and after divv_switch only;
+ * other steps return QSM_ERR_STATE. */
 int qsm_get_field(qsm_handle h, int which, double* host);
 /* diagnostics of the current state (compensated FP64 sums over the fields
  * of the last step); QSM_ERR_NONFINITE if any is NaN/Inf */

@@ -294,6 +294,8 @@ struct qsm_handle_s {
   double* psihat_old = nullptr;
   float* psihat32 = nullptr;  // mixed precision: psihat rounded to FP32 (the spectral prologues' source)
   bool divv_valid = false;    // the D field holds IDCT(scale D-hat) of the current step
+  bool p_valid = false;       // the p field holds the current step's pressure
+  bool r_valid = false;       // the grad rho fields hold the current step's (lagged) gradient
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
   double* dsc = nullptr;
@@ -541,6 +543,19 @@ const void* psihat_src(const qsm_handle_s* h) {
   return h->fp32 ? (const void*)h->psihat32 : (const void*)h->psihat;
 }
 
+// Whether the grad rho arrays hold the gradient that step n reads: steps
+// after divv_switch form it; at step 1 it is that of rhohat = 0, the zeros
+// the arrays hold from set-up; other steps do not form it (quasi_step).
+bool grad_rho_formed(const qsm_params& p, long long n) { return n > p.divv_switch || n <= 1; }
+
+// p-hat from the current S-hat (quasi.cpp:1385-1415), as the load of p's x pass
+LoadOp pressure_load(qsm_handle_s* h) {
+  LoadOp L{};
+  L.code = LD_PRESSURE;
+  L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
+  return L;
+}
+
 int quasi_step(qsm_handle_s* h) {
   const long long n = ++h->step;
   const qsm_params& p = h->p;
@@ -550,7 +565,16 @@ int quasi_step(qsm_handle_s* h) {
   void *rho = F(h, h->rho_cur), *rho_old = F(h, h->rho_old), *psi = F(h, h->psi_cur);
   const void* psih = psihat_src(h);
   const bool dlive = n > p.divv_switch;  // D is computed from this step on
+  // grad rho is read by D (n > divv_switch) and by N's pressure term
+  // (n > divv_switch + 1); p only by that term (quasi.cpp:1273-1293,
+  // 1905-1939).  Before that they are read by no stage of the step, so they
+  // are not formed: grad rho at step 1 is that of rhohat = 0, which its
+  // arrays hold from set-up, and p is formed when the field is asked for
+  // (pressure_output).
+  const bool plive = n > p.divv_switch + 1;
   h->divv_valid = false;
+  h->p_valid = plive;
+  h->r_valid = grad_rho_formed(p, n);
 
   const bool grp = !h->rec;  // group kernels (single handles)
   LoadOp L{};
@@ -570,13 +594,15 @@ int quasi_step(qsm_handle_s* h) {
     }
     for (auto& x : it) { x.L.a[0] = psih; x.L.axis = x.dst_axis; }
     if ((rc = inv3_group(h, GRP_PSI, it, 8, 1))) return rc;
-    for (int c = 0; c < 3; ++c) {
-      it[c] = {LoadOp{}, F(h, F_RX + c), c};
-      it[c].L.code = LD_GRAD_X + c;
-      it[c].L.axis = c;
-      it[c].L.a[0] = F(h, F_RHOHAT);
+    if (dlive) {
+      for (int c = 0; c < 3; ++c) {
+        it[c] = {LoadOp{}, F(h, F_RX + c), c};
+        it[c].L.code = LD_GRAD_X + c;
+        it[c].L.axis = c;
+        it[c].L.a[0] = F(h, F_RHOHAT);
+      }
+      if ((rc = inv3_group(h, GRP_RHO, it, 3, 1))) return rc;
     }
-    if ((rc = inv3_group(h, GRP_RHO, it, 3, 1))) return rc;
   } else {
     // S1: F, G (quasi.cpp:1034-1057)
     L.code = LD_COEF_F; L.a[0] = psih;
@@ -588,8 +614,10 @@ int quasi_step(qsm_handle_s* h) {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
       if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
-      if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
+      if (dlive) {
+        L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
+        if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
+      }
       L = LoadOp{}; L.code = LD_LAPGRAD_X + c; L.axis = c; L.a[0] = psih;
       if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
     }
@@ -647,11 +675,11 @@ int quasi_step(qsm_handle_s* h) {
       it[1 + c].L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
     }
     for (auto& x : it) { x.L.a[0] = F(h, F_SX); x.L.a[1] = F(h, F_SY); x.L.a[2] = F(h, F_SZ); }
-    if ((rc = inv3_group(h, GRP_PROJ, it, 4, dlive ? 4 : 3))) return rc;
+    if (plive) rc = inv3_group(h, GRP_PROJ, it, 4, 4);
+    else rc = inv3_group(h, GRP_VEL, it + 1, 3, dlive ? 4 : 3);
+    if (rc) return rc;
   } else {
-    L = LoadOp{}; L.code = LD_PRESSURE;
-    L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
-    if ((rc = inv3(h, -1, L, F(h, F_P)))) return rc;
+    if (plive && (rc = inv3(h, -1, pressure_load(h), F(h, F_P)))) return rc;
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_VEL0 + c; L.n0 = n0;
       L.a[0] = F(h, F_SX); L.a[1] = F(h, F_SY); L.a[2] = F(h, F_SZ);
@@ -665,7 +693,7 @@ int quasi_step(qsm_handle_s* h) {
   // (divv_output).
 
   // S9 + S10: N, its transform, and the update fused into the last pass
-  L = LoadOp{}; L.code = n > p.divv_switch + 1 ? LD_NONLIN_P : LD_NONLIN;
+  L = LoadOp{}; L.code = plive ? LD_NONLIN_P : LD_NONLIN;
   L.a[0] = rho; L.a[1] = psi; L.a[2] = F(h, F_F); L.a[3] = F(h, F_G);
   L.a[4] = F(h, F_VX); L.a[5] = F(h, F_VY); L.a[6] = F(h, F_VZ);
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
@@ -690,6 +718,17 @@ int divv_output(qsm_handle_s* h) {
   L.code = LD_SCALE; L.a[0] = F(h, F_DHAT);
   const int rc = inv3(h, -1, L, F(h, F_DIVV));
   if (!rc) h->divv_valid = true;
+  return rc;
+}
+
+// The p field of the current step, formed on request when the step did not
+// need it (steps <= divv_switch + 1): IDCT of p-hat from the S-hat arrays,
+// which hold this step's stress transforms until the next step's S5.  The
+// same x pass as the projection group's first output, so the same bits.
+int pressure_output(qsm_handle_s* h) {
+  if (h->p_valid) return QSM_OK;
+  const int rc = inv3(h, -1, pressure_load(h), F(h, F_P));
+  if (!rc) h->p_valid = true;
   return rc;
 }
 
@@ -732,7 +771,7 @@ int stb_step(qsm_handle_s* h) {
       it[c].L.a[0] = F(h, F_SX); it[c].L.a[1] = F(h, F_SY); it[c].L.a[2] = F(h, F_SZ);
       it[c].L.a[3] = nullptr;
     }
-    if ((rc = inv3_group(h, GRP_STB_PROJ, it, 3, 3))) return rc;
+    if ((rc = inv3_group(h, GRP_VEL, it, 3, 3))) return rc;
   } else {
     L.code = LD_SH; L.a[0] = psih;
     if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
@@ -960,6 +999,8 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   h->step = 0;
   h->growth = 0;
   h->divv_valid = false;
+  h->p_valid = false;
+  h->r_valid = true;
   h->ready = true;
   return QSM_OK;
 }
@@ -1092,7 +1133,14 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
     case QSM_VX: src = F(h, F_VX); break;
     case QSM_VY: src = F(h, F_VY); break;
     case QSM_VZ: src = F(h, F_VZ); break;
-    case QSM_P: src = F(h, F_P); break;
+    case QSM_P: {
+      if (h->p.variant == QSM_QUASI) {
+        const int rc = pressure_output(h);
+        if (rc) return rc;
+      }
+      src = F(h, F_P);
+      break;
+    }
     case QSM_DIVV: {
       const int rc = divv_output(h);
       if (rc) return rc;
@@ -1104,7 +1152,13 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
     case QSM_F: src = F(h, F_F); break;
     case QSM_G: src = F(h, F_G); break;
     case QSM_GX: case QSM_GY: case QSM_GZ: src = F(h, F_GX + (which - QSM_GX)); break;
-    case QSM_RX: case QSM_RY: case QSM_RZ: src = F(h, F_RX + (which - QSM_RX)); break;
+    case QSM_RX: case QSM_RY: case QSM_RZ:
+      if (h->p.variant == QSM_QUASI && !h->r_valid)
+        return fail(QSM_ERR_STATE,
+                    "qsm_get_field: grad rho is formed at step 1 and after divv_switch only (step %lld)",
+                    h->step);
+      src = F(h, F_RX + (which - QSM_RX));
+      break;
     case QSM_LX: case QSM_LY: case QSM_LZ: src = F(h, F_LX + (which - QSM_LX)); break;
     case QSM_RHOHAT: src = F(h, F_RHOHAT); break;
     case QSM_NHAT: src = F(h, F_NHAT_OLD); break;
@@ -1514,6 +1568,8 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)h->n, h->stream));
   h->step = hd.step;
   h->divv_valid = false;
+  h->p_valid = false;
+  h->r_valid = grad_rho_formed(h->p, h->step);
   h->ready = true;
   if (h->p.variant == QSM_STB) return stb_growth(h);
   return QSM_OK;

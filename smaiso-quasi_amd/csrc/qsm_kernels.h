@@ -63,7 +63,7 @@ enum GroupCode : int {
   GRP_RHO,         // grad rho (x, y, z) from rhohat
   GRP_PROJ,        // p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
   GRP_STB_PSI,     // sh, grad psi (x, y, z)
-  GRP_STB_PROJ,    // v_x, v_y, v_z
+  GRP_VEL,         // v_x, v_y, v_z (stb, and quasi steps whose p is read by nothing)
   GRP_COUNT
 };
 constexpr int kMaxGroup = 8;

@@ -984,7 +984,7 @@ struct GroupOps<GRP_STB_PSI> {
   static constexpr int VAR64 = 0;
 };
 template <>
-struct GroupOps<GRP_STB_PROJ> {
+struct GroupOps<GRP_VEL> {
   using L = OpList<LD_VEL0, LD_VEL1, LD_VEL2>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
@@ -1156,7 +1156,7 @@ hipError_t launch_group_n(int group, const GroupArgs& G, long* nblocks, hipStrea
     case GRP_RHO: return launch_group_g<T, N, AX, GRP_RHO>(G, nblocks, s);
     case GRP_PROJ: return launch_group_g<T, N, AX, GRP_PROJ>(G, nblocks, s);
     case GRP_STB_PSI: return launch_group_g<T, N, AX, GRP_STB_PSI>(G, nblocks, s);
-    case GRP_STB_PROJ: return launch_group_g<T, N, AX, GRP_STB_PROJ>(G, nblocks, s);
+    case GRP_VEL: return launch_group_g<T, N, AX, GRP_VEL>(G, nblocks, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -142,6 +142,48 @@ def test_one_step_all_fields_fp64(qsm_mod, oracle_mod, compat):
         _cmp_fields(sim, orc, STEP_FIELDS, 1e-10, f"compat{compat} step{s + 1}", tols)
 
 
+@pytest.mark.parametrize("prec", (0, 1))
+def test_fields_formed_on_request(qsm_mod, oracle_mod, prec):
+    """Up to step divv_switch + 1 no stage reads p, and up to divv_switch none
+    reads grad rho, so the step does not form them.  p is formed on request
+    from the step's S-hat and equals the oracle's (FP64 1e-10; mixed 1e-4,
+    fields formed from FP32 stress transforms);
+    grad rho is zero at step 1 (rhohat = 0), an error at steps 2..divv_switch
+    and the oracle's after.  Reading p on request leaves the trajectory bit
+    for bit unchanged."""
+    shape = (32, 48, 64)
+    tol = 1e-10 if prec == 0 else 1e-4
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=shape, prec=prec, divv_switch=4)
+    quiet, _ = _pair(qsm_mod, oracle_mod, shape=shape, prec=prec, divv_switch=4)
+    for n in range(1, 7):
+        sim.step(1)
+        quiet.step(1)
+        orc.step(1)
+        if n <= 5:  # before v (amplified ~1e6 once D is live) reaches psi
+            e = rel_l2(sim.field("P"), orc.field("P"))
+            print(f"prec {prec} step {n} P: rel-L2 {e:.2e}", flush=True)
+            assert e < tol, (n, e)
+        if n == 1:
+            assert not np.any(sim.field("RX")) and not np.any(orc.field("RX"))
+        elif n <= 4:
+            with pytest.raises(qsm_mod.QsmError):
+                sim.field("RY")
+        elif n == 5:
+            # FP64 per component.  Mixed: the vector as a whole, since psi
+            # here varies in x and y only through its 1e-4 noise, and the x
+            # and y components of grad rho (~1e-4 of rho's range) are at the
+            # resolution of FP32 rho (~1)
+            got = np.stack([sim.field(f) for f in ("RX", "RY", "RZ")])
+            ref = np.stack([orc.field(f) for f in ("RX", "RY", "RZ")])
+            errs = [rel_l2(got, ref)] if prec else [rel_l2(a, b) for a, b in zip(got, ref)]
+            print(f"prec {prec} step {n} grad rho: rel-L2 {errs}", flush=True)
+            assert max(errs) < tol, (n, errs)
+    for f in ("PSI", "PSIHAT", "RHO", "VX", "VY", "VZ", "NHAT", "P"):
+        assert np.array_equal(sim.field(f), quiet.field(f)), f
+    sim.close()
+    quiet.close()
+
+
 @pytest.mark.parametrize("shape", [(1024, 32, 32), (32, 1024, 32), (32, 32, 1024), (96, 128, 64)])
 @pytest.mark.parametrize("prec", (0, 1))
 def test_step_long_axes(qsm_mod, oracle_mod, shape, prec):
