@@ -1258,6 +1258,7 @@ namespace {
 
 struct EnsLaunch {
   int kind = 0, axis = 0, n = 0, inv = 0, ld = 0, st = 0, count = 0, variant = 0;
+  int q = 0;                  // launch of the instances' step sequence this serves
   long total = 0;
   PassArgs* descs = nullptr;  // device
   int* prefix = nullptr;      // device
@@ -1269,8 +1270,14 @@ struct EnsLaunch {
 struct EnsPlan {
   std::vector<EnsLaunch> launches;
   std::vector<void*> allocs;
+  std::vector<hipEvent_t> events;  // fork and join events, one per launch
   hipGraphExec_t exec = nullptr;
 };
+
+// Side streams of an ensemble: the launches of one pass for different pencil
+// lengths touch disjoint instances, so up to kEnsSide + 1 of them run
+// side by side.  Default priority, like every stream here.
+constexpr int kEnsSide = 4;
 
 }  // namespace
 
@@ -1278,6 +1285,7 @@ struct qsm_ens_s {
   std::vector<qsm_handle_s*> inst;
   int prec = 0, fp32 = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side[kEnsSide] = {};
   bool use_graphs = true;
   std::map<std::vector<long long>, EnsPlan*> plans;
   std::map<int, int> variant[3][2];  // tile-width variant of plain passes [axis][inv][length]
@@ -1295,18 +1303,40 @@ int upload(EnsPlan* pl, const std::vector<X>& v, X** out) {
   return QSM_OK;
 }
 
+// one batched launch of a plan on stream s
+hipError_t ens_launch(const qsm_ens_s* e, const EnsLaunch& L, hipStream_t s) {
+  if (L.kind == 0) {
+    BatchLaunch B{L.descs, L.prefix, L.count, L.total};
+    PassArgs dummy{};
+    return launch_pass(e->fp32, L.n, L.axis, L.inv != 0, L.ld, L.st, dummy, nullptr, s, &B,
+                       L.variant);
+  }
+  return launch_finish_batch(L.parts, L.nparts, L.count, L.outs, s);
+}
+
+// The launches of one step, in order.  Those serving one launch q of the
+// instances' step sequence (one per pencil length) touch disjoint instances:
+// the first runs on the ensemble stream, up to kEnsSide more on side streams
+// forked from it, and all are joined back before the next launch starts.
 int ens_issue(qsm_ens_s* e, const EnsPlan* pl) {
-  for (const EnsLaunch& L : pl->launches) {
-    hipError_t err;
-    if (L.kind == 0) {
-      BatchLaunch B{L.descs, L.prefix, L.count, L.total};
-      PassArgs dummy{};
-      err = launch_pass(e->fp32, L.n, L.axis, L.inv != 0, L.ld, L.st, dummy, nullptr, e->stream, &B,
-                        L.variant);
-    } else {
-      err = launch_finish_batch(L.parts, L.nparts, L.count, L.outs, e->stream);
+  const size_t nl = pl->launches.size();
+  size_t a = 0;
+  while (a < nl) {
+    size_t b = a + 1;
+    while (b < nl && pl->launches[b].q == pl->launches[a].q && b - a <= (size_t)kEnsSide) ++b;
+    hipError_t err = hipSuccess;
+    if (b - a > 1) err = hipEventRecord(pl->events[a], e->stream);  // fork
+    for (size_t i = a + 1; i < b && err == hipSuccess; ++i) {
+      const hipStream_t side = e->side[i - a - 1];
+      err = hipStreamWaitEvent(side, pl->events[a], 0);
+      if (err == hipSuccess) err = ens_launch(e, pl->launches[i], side);
+      if (err == hipSuccess) err = hipEventRecord(pl->events[i], side);  // join
     }
+    if (err == hipSuccess) err = ens_launch(e, pl->launches[a], e->stream);
+    for (size_t i = a + 1; i < b && err == hipSuccess; ++i)
+      err = hipStreamWaitEvent(e->stream, pl->events[i], 0);
     if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble launch: %s", hipGetErrorString(err));
+    a = b;
   }
   return QSM_OK;
 }
@@ -1337,7 +1367,7 @@ int ens_build(qsm_ens_s* e, EnsPlan* pl) {
       std::vector<double*> outs(count);
       for (int i = 0; i < count; ++i) { parts[i] = recs[i][q].part; np[i] = recs[i][q].nparts; outs[i] = recs[i][q].out; }
       EnsLaunch L;
-      L.kind = 1; L.count = count;
+      L.kind = 1; L.count = count; L.q = (int)q;
       int rc = upload(pl, parts, &L.parts);
       if (!rc) rc = upload(pl, np, &L.nparts);
       if (!rc) rc = upload(pl, outs, &L.outs);
@@ -1359,6 +1389,7 @@ int ens_build(qsm_ens_s* e, EnsPlan* pl) {
       }
       EnsLaunch L;
       L.kind = 0; L.axis = r0.axis; L.n = g.first; L.inv = r0.inv; L.ld = r0.ld; L.st = r0.st;
+      L.q = (int)q;
       L.variant = var;
       L.count = (int)g.second.size(); L.total = prefix.back();
       int rc = upload(pl, descs, &L.descs);
@@ -1367,11 +1398,17 @@ int ens_build(qsm_ens_s* e, EnsPlan* pl) {
       pl->launches.push_back(L);
     }
   }
+  for (size_t i = 0; i < pl->launches.size(); ++i) {
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    pl->events.push_back(ev);
+  }
   return QSM_OK;
 }
 
 void ens_free_plan(EnsPlan* pl) {
   if (pl->exec) hipGraphExecDestroy(pl->exec);
+  for (hipEvent_t ev : pl->events) hipEventDestroy(ev);
   for (void* d : pl->allocs) hipFree(d);
   delete pl;
 }
@@ -1427,6 +1464,8 @@ extern "C" int qsm_ens_create(const qsm_params* params, int count, int prec, qsm
     e->inst.push_back(h);
   }
   if (hipStreamCreate(&e->stream) != hipSuccess) { qsm_ens_destroy(e); return fail(QSM_ERR_HIP, "stream"); }
+  for (auto& side : e->side)
+    if (hipStreamCreate(&side) != hipSuccess) { qsm_ens_destroy(e); return fail(QSM_ERR_HIP, "stream"); }
   *out = e;
   return QSM_OK;
 }
@@ -1484,6 +1523,8 @@ extern "C" int qsm_ens_destroy(qsm_ens e) {
   if (e->stream) hipStreamSynchronize(e->stream);
   for (auto& kv : e->plans) ens_free_plan(kv.second);
   for (auto* h : e->inst) qsm_destroy(h);
+  for (auto side : e->side)
+    if (side) { hipStreamSynchronize(side); hipStreamDestroy(side); }
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return QSM_OK;

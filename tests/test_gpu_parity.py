@@ -275,12 +275,15 @@ def test_graph_replay_matches_eager(qsm_mod, prec):
         assert g.diag() == e.diag()
 
 
+@pytest.mark.parametrize("graphs", (True, False))
 @pytest.mark.parametrize("prec", (0, 1))
-def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec):
+def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec, graphs):
     """A batched ensemble of stb instances with mixed grid edges gives, for
     every instance, the fields and growth rate of that instance run alone
     through the single-instance path bit for bit, and FP64 members match the
-    oracle.  The batched path runs one transform per launch; the single path
+    oracle; with the steps replayed from captured graphs and launched
+    eagerly, the launches of one pass for different pencil lengths forked
+    onto side streams either way.  The batched path runs one transform per launch; the single path
     runs the transforms that share inputs as group and multi-output passes.
     Both evaluate every output with the same operator expression and the
     same transform code in the same axis order, so no rounding differs."""
@@ -288,6 +291,7 @@ def test_ensemble_matches_single_runs(qsm_mod, oracle_mod, prec):
     members = ics.ensemble_members(count=6, seed=11)
     params = [qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_}) for d, _ in members]
     ens = qsm_mod.Ensemble(params, prec)
+    ens.graphs(graphs)
     for i, (_, psi) in enumerate(members):
         ens.set_psi(i, psi)
     ens.step(2)
