@@ -50,7 +50,7 @@ LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
             "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3"]
 # timer categories 40 + g of the x-pass group kernels (one kernel, several outputs)
-GROUPS = ["psi", "rho", "proj", "stb_psi", "stb_proj"]
+GROUPS = ["psi", "rho", "proj", "stb_psi", "vel"]
 
 
 def load_op_name(code):
