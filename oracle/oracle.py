@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, as the checker.  Nothing in the product library
 imports it.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -18,7 +19,7 @@ REDFT01, REDFT10, RODFT01, RODFT10 = 4, 5, 8, 9
 FIELDS = [
     "PSI", "RHO", "VX", "VY", "VZ", "P", "DIVV", "PSIHAT", "F", "G", "GX", "GY", "GZ",
     "RX", "RY", "RZ", "LX", "LY", "LZ", "RHOHAT", "NHAT", "PSIHAT_OLD", "PSI_OLD",
-    "SX", "SY", "SZ", "DHAT", "MU",
+    "SX", "SY", "SZ", "DHAT", "MU", "DTPSI", "RHO_OLD",
 ]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 
@@ -43,33 +44,63 @@ def build():
 _lib = None
 
 
+def load_library(path):
+    """Load an oracle build (this directory's, or a test's variant of it) and
+    declare its signatures."""
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    pp = ctypes.POINTER(OrcParams)
+    sz = ctypes.c_size_t
+    L.orc_r2r_1d.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+    L.orc_r2r_1d_direct.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+    L.orc_r2r_3d.argtypes = [ctypes.c_int] * 6 + [dp, dp]
+    L.orc_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
+    L.orc_spec_grad.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp, ctypes.c_int, dp, dp]
+    L.orc_halo_sx_slab.argtypes = [ctypes.c_int] * 3
+    L.orc_project.argtypes = [pp] + [dp] * 8
+    L.orc_density_law.argtypes = [pp, sz] + [dp] * 5
+    L.orc_density_filter.argtypes = [pp, dp, dp, dp]
+    L.orc_divergence.argtypes = [pp, sz] + [dp] * 9
+    L.orc_stress.argtypes = [sz] + [dp] * 12
+    L.orc_nonlinear.argtypes = [pp, sz, ctypes.c_int] + [dp] * 15
+    L.orc_stb_stress.argtypes = [pp, sz] + [dp] * 9
+    L.orc_stb_nonlinear.argtypes = [pp, sz] + [dp] * 10
+    L.orc_stb_growth_mode.argtypes = [pp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.orc_create.argtypes = [pp]
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_set_psi.argtypes = [ctypes.c_void_p, dp]
+    L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+    L.orc_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+    L.orc_diag.argtypes = [ctypes.c_void_p, dp]
+    L.orc_get_step.argtypes = [ctypes.c_void_p]
+    L.orc_get_growth.argtypes = [ctypes.c_void_p]
+    L.orc_get_growth.restype = ctypes.c_double
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        dp = ctypes.POINTER(ctypes.c_double)
-        L.orc_r2r_1d.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
-        L.orc_r2r_1d_direct.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
-        L.orc_r2r_3d.argtypes = [ctypes.c_int] * 6 + [dp, dp]
-        L.orc_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
-        L.orc_spec_grad.argtypes = [ctypes.c_int] * 3 + [dp, dp, dp, ctypes.c_int, dp, dp]
-        L.orc_halo_sx_slab.argtypes = [ctypes.c_int] * 3
-        L.orc_project.argtypes = [ctypes.POINTER(OrcParams)] + [dp] * 8
-        L.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
-        L.orc_create.restype = ctypes.c_void_p
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_set_psi.argtypes = [ctypes.c_void_p, dp]
-        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
-        L.orc_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
-        L.orc_diag.argtypes = [ctypes.c_void_p, dp]
-        L.orc_get_step.argtypes = [ctypes.c_void_p]
-        L.orc_get_growth.argtypes = [ctypes.c_void_p]
-        L.orc_get_growth.restype = ctypes.c_double
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using_library(path):
+    """Run the enclosed calls against another build of the oracle (the
+    mutation tests plant a mistake in a copy of qsm_oracle.c and check that a
+    pin catches it)."""
+    global _lib
+    saved = lib()
+    _lib = load_library(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def _p(a):
@@ -133,6 +164,87 @@ def project(params, Sx, Sy, Sz, dhat):
     lib().orc_project(ctypes.byref(params), *[_p(a) for a in arrs], _opt(dh),
                       *[_p(o) for o in outs])
     return outs  # phat, vxh, vyh, vzh
+
+
+def _arr(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else np.ascontiguousarray(np.broadcast_to(a, shape))
+
+
+def _fields(shape, *arrs):
+    return [_arr(a, shape) for a in arrs]
+
+
+def density_law(params, psi, g):
+    """rho* of quasi.cpp:1237-1247 from psi and grad psi (three arrays)."""
+    psi, gx, gy, gz = _fields(np.shape(psi), psi, *g)
+    out = np.empty_like(psi)
+    lib().orc_density_law(ctypes.byref(params), psi.size, _p(psi), _p(gx), _p(gy), _p(gz), _p(out))
+    return out
+
+
+def density_filter(params, rho_star):
+    """(rhohat, rho) of quasi.cpp:1249-1268."""
+    rs = _arr(rho_star)
+    rh, rho = np.empty_like(rs), np.empty_like(rs)
+    lib().orc_density_filter(ctypes.byref(params), _p(rs), _p(rh), _p(rho))
+    return rh, rho
+
+
+def divergence(params, rho, rho_old, v, r):
+    """D of quasi.cpp:1273-1287 (real space, before its transform)."""
+    shape = np.shape(rho)
+    a = _fields(shape, rho, rho_old, *v, *r)
+    out = np.empty(shape)
+    lib().orc_divergence(ctypes.byref(params), out.size, *[_p(x) for x in a], _p(out))
+    return out
+
+
+def stress(rho, G, F, g, l):
+    """(S_x, S_y, S_z, G <- rho G) of quasi.cpp:1298-1317."""
+    shape = np.shape(rho)
+    rho, F, gx, gy, gz, lx, ly, lz = _fields(shape, rho, F, *g, *l)
+    Gc = _arr(G, shape).copy()
+    S = [np.empty(shape) for _ in range(3)]
+    lib().orc_stress(rho.size, _p(rho), _p(Gc), _p(F), _p(gx), _p(gy), _p(gz), _p(lx), _p(ly),
+                     _p(lz), *[_p(x) for x in S])
+    return S, Gc
+
+
+def nonlinear(params, step, rho, psi, F, G, v, g, P, r):
+    """N of quasi.cpp:1905-1931 at step `step` (real space)."""
+    shape = np.shape(rho)
+    a = _fields(shape, rho, psi, F, G, *v, *g, P, *r)
+    out = np.empty(shape)
+    lib().orc_nonlinear(ctypes.byref(params), out.size, step, *[_p(x) for x in a], _p(out))
+    return out
+
+
+def stb_stress(params, sh, psi, g):
+    """(mu, [S_x, S_y, S_z]) of quasi-stb.cpp:1284-1307."""
+    shape = np.shape(sh)
+    a = _fields(shape, sh, psi, *g)
+    mu = np.empty(shape)
+    S = [np.empty(shape) for _ in range(3)]
+    lib().orc_stb_stress(ctypes.byref(params), mu.size, *[_p(x) for x in a], _p(mu),
+                         *[_p(x) for x in S])
+    return mu, S
+
+
+def stb_nonlinear(params, psi, mu, v, g):
+    """(N, dtpsi) of quasi-stb.cpp:1893-1923."""
+    shape = np.shape(psi)
+    a = _fields(shape, psi, mu, *v, *g)
+    N, dt = np.empty(shape), np.empty(shape)
+    lib().orc_stb_nonlinear(ctypes.byref(params), N.size, *[_p(x) for x in a], _p(N), _p(dt))
+    return N, dt
+
+
+def stb_growth_mode(params):
+    """(i*, k*) of quasi-stb.cpp:2147-2150, or None off the grid."""
+    i, k = ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_stb_growth_mode(ctypes.byref(params), ctypes.byref(i), ctypes.byref(k))
+    return (i.value, k.value) if rc == 0 else None
 
 
 class State:

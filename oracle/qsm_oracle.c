@@ -22,11 +22,21 @@
  *     velocity stencils of v_y, v_z take S_x of slab 0 of the previous rank
  *     instead of its last slab), reproduced when compat_ranks = P > 0.
  *
- * Pins (tests/test_oracle_*.py):
+ * Pins (tests/test_oracle_transforms.py, test_oracle_step.py, test_oracle_pins.py
+ * with the checks in tests/pins.py; tests/test_oracle_mutations.py plants one
+ * mistake per stage in a copy of this file and shows that its pin fails):
  *   r2r fast path      vs O(N^2) direct definitions and SciPy golden vectors
  *   3D r2r             separability vs 1D, round trip 2N per axis
  *   shifted gradient   closed-form derivative of the cosine interpolant
- *   density law        single-mode psi gives constant rho (closed form)
+ *   density law        Pythagorean |(q0 psi, g)|; a mode at q0 gives constant rho
+ *   density filter     one long cosine mode scaled by exp(-1.57*1.57*8 K^2)
+ *   divergence         continuity: uniform growth -(2/dt) tanh(lambda dt/2),
+ *                      advected linear profile 0; rho_old, grad rho and v lags
+ *   stress             single mode S = 0; two-mode closed form; G <- rho G
+ *   nonlinear N        Chebyshev harmonics at q0; -F - rho G off q0; v.g for
+ *                      orthogonal/parallel vectors; pressure term and its gate
+ *   stb                local mu -> v = 0; harmonics; dtpsi vs the update;
+ *                      growth-rate mode and Crank-Nicolson ratio
  *   projection         gradient S -> v = 0, p = phi; solenoidal S -> p = 0;
  *                      S = 0 -> div v = D on interior modes
  *   update             linear-regime two-root recurrence (closed form)
@@ -416,6 +426,136 @@ void orc_project(const orc_params *pr, const double *Sx, const double *Sy, const
 }
 
 /* ------------------------------------------------------------------ */
+/* pointwise stages of the step (each exported so that the tests can    */
+/* feed it prescribed fields; the step below composes them in the       */
+/* reference's order)                                                   */
+/* ------------------------------------------------------------------ */
+
+/* quasi.cpp:1237-1247: rho* = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 */
+void orc_density_law(const orc_params *p, size_t n, const double *psi, const double *gx,
+                     const double *gy, const double *gz, double *rho_star) {
+  const double q02 = p->q0 * p->q0;
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t)
+    rho_star[t] = p->kp * sqrt(q02 * pow(psi[t], 2) + pow(gx[t], 2) + pow(gy[t], 2) +
+                               pow(gz[t], 2)) + p->rho0;
+}
+
+/* quasi.cpp:1249-1268: rhohat = scale DCT(rho*) exp(-filter K^2), rho = IDCT rhohat.
+ * rho_star is left unchanged. */
+void orc_density_filter(const orc_params *p, const double *rho_star, double *rhohat,
+                        double *rho) {
+  int nx = p->nx, ny = p->ny, nz = p->nz;
+  size_t n = (size_t)nx * ny * nz;
+  double sc = 0.125 / ((double)n);
+  double *qx = malloc(sizeof(double) * nx), *sx = malloc(sizeof(double) * nx);
+  double *qy = malloc(sizeof(double) * ny), *sy = malloc(sizeof(double) * ny);
+  double *qz = malloc(sizeof(double) * nz), *sz = malloc(sizeof(double) * nz);
+  orc_wavenumbers(nx, nx * 2.0 * M_PI / (p->q0 * p->nw), qx, sx);
+  orc_wavenumbers(ny, ny * 2.0 * M_PI / (p->q0 * p->nw), qy, sy);
+  orc_wavenumbers(nz, nz * 2.0 * M_PI / (p->q0 * p->nw), qz, sz);
+  memcpy(rhohat, rho_star, n * sizeof(double));
+  r2r3d_inplace(nx, ny, nz, ORC_REDFT10, ORC_REDFT10, ORC_REDFT10, rhohat);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) {
+    size_t id = IDX(i, j, k);
+    double mq2 = pow(qx[i], 2) + pow(qy[j], 2) + pow(qz[k], 2);
+    rhohat[id] = sc * rhohat[id] * exp(-p->filter * mq2);
+  }
+  memcpy(rho, rhohat, n * sizeof(double));
+  r2r3d_inplace(nx, ny, nz, ORC_REDFT01, ORC_REDFT01, ORC_REDFT01, rho);
+  free(qx); free(sx); free(qy); free(sy); free(qz); free(sz);
+}
+
+/* quasi.cpp:1273-1287: D = -((rho - rho_old)/dt + v.r)/(0.5 (rho + rho_old)) */
+void orc_divergence(const orc_params *p, size_t n, const double *rho, const double *rho_old,
+                    const double *vx, const double *vy, const double *vz, const double *rx,
+                    const double *ry, const double *rz, double *D) {
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t)
+    D[t] = -((rho[t] - rho_old[t]) / p->dt + vx[t] * rx[t] + vy[t] * ry[t] + vz[t] * rz[t]) /
+           (0.5 * (rho[t] + rho_old[t]));
+}
+
+/* quasi.cpp:1298-1317: G <- rho G (kept: N reads it), S_c = G g_c - rho F l_c */
+void orc_stress(size_t n, const double *rho, double *G, const double *F, const double *gx,
+                const double *gy, const double *gz, const double *lx, const double *ly,
+                const double *lz, double *Sx, double *Sy, double *Sz) {
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t) {
+    double g = rho[t] * G[t];
+    G[t] = g;
+    double rf = rho[t] * F[t];
+    Sx[t] = g * gx[t] - rf * lx[t];
+    Sy[t] = g * gy[t] - rf * ly[t];
+    Sz[t] = g * gz[t] - rf * lz[t];
+  }
+}
+
+/* quasi.cpp:1905-1931: N = rho (eps psi - F + beta psi^3 - gamma psi^5) - G - v.g,
+ * plus p kp |grad rho| / rho_s^2 (rho_s = kp Amp + rho0) from step divvSwitch + 2 */
+void orc_nonlinear(const orc_params *p, size_t n, int step, const double *rho,
+                   const double *psi, const double *F, const double *G, const double *vx,
+                   const double *vy, const double *vz, const double *gx, const double *gy,
+                   const double *gz, const double *P, const double *rx, const double *ry,
+                   const double *rz, double *N) {
+  double rs2 = pow(p->kp * p->amp + p->rho0, 2);
+  int with_p = step > p->divv_switch + 1;
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t) {
+    double ps = psi[t];
+    double v = rho[t] * (p->eps * ps - F[t] + p->beta * pow(ps, 3) - p->gamma * pow(ps, 5)) -
+               G[t] - vx[t] * gx[t] - vy[t] * gy[t] - vz[t] * gz[t];
+    if (with_p) v += P[t] * p->kp * sqrt(pow(rx[t], 2) + pow(ry[t], 2) + pow(rz[t], 2)) / rs2;
+    N[t] = v;
+  }
+}
+
+/* quasi-stb.cpp:1284-1307: mu = sh - beta psi^3 + gamma psi^5, S_c = rho mu g_c (rho = 1) */
+void orc_stb_stress(const orc_params *p, size_t n, const double *sh, const double *psi,
+                    const double *gx, const double *gy, const double *gz, double *mu_out,
+                    double *Sx, double *Sy, double *Sz) {
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t) {
+    double rho = 1;
+    double mu = sh[t] - p->beta * pow(psi[t], 3) + p->gamma * pow(psi[t], 5);
+    mu_out[t] = mu;
+    Sx[t] = rho * mu * gx[t];
+    Sy[t] = rho * mu * gy[t];
+    Sz[t] = rho * mu * gz[t];
+  }
+}
+
+/* quasi-stb.cpp:1893-1923: N = beta psi^3 - gamma psi^5 [- v.g if hydro == 1] and the
+ * time derivative written to dtpsi.dat, dtpsi = -rho mu [- v.g] (rho = 1) */
+void orc_stb_nonlinear(const orc_params *p, size_t n, const double *psi, const double *mu,
+                       const double *vx, const double *vy, const double *vz, const double *gx,
+                       const double *gy, const double *gz, double *N, double *dtpsi) {
+#pragma omp parallel for schedule(static)
+  for (size_t t = 0; t < n; ++t) {
+    double rho = 1;
+    double adv = vx[t] * gx[t] + vy[t] * gy[t] + vz[t] * gz[t];
+    double nl = p->beta * pow(psi[t], 3) - p->gamma * pow(psi[t], 5);
+    double dp = -rho * mu[t];
+    if (p->hydro == 1) { nl -= adv; dp -= adv; }
+    N[t] = nl;
+    dtpsi[t] = dp;
+  }
+}
+
+/* quasi-stb.cpp:2147-2150: the mode (i*, 0, k*) of the growth rate, i* =
+ * round(Qi / q[1]), k* = round(q0 / q_z[1]).  The reference takes q_z[1] for
+ * both (its grids are cubic); here i* uses the x table, the same value on a
+ * cubic grid.  Returns 0 when the mode lies on the grid. */
+int orc_stb_growth_mode(const orc_params *p, int *is, int *ks) {
+  double qx1 = M_PI / (p->nx * 2.0 * M_PI / (p->q0 * p->nw));
+  double qz1 = M_PI / (p->nz * 2.0 * M_PI / (p->q0 * p->nw));
+  *is = (int)lround(p->growth_q / qx1);
+  *ks = (int)lround(p->q0 / qz1);
+  return (*is < p->nx && *ks < p->nz) ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------ */
 /* state and step                                                       */
 /* ------------------------------------------------------------------ */
 
@@ -423,7 +563,7 @@ enum {
   ORC_PSI = 0, ORC_RHO, ORC_VX, ORC_VY, ORC_VZ, ORC_P, ORC_DIVV, ORC_PSIHAT,
   ORC_F, ORC_G, ORC_GX, ORC_GY, ORC_GZ, ORC_RX, ORC_RY, ORC_RZ, ORC_LX, ORC_LY, ORC_LZ,
   ORC_RHOHAT, ORC_NHAT, ORC_PSIHAT_OLD, ORC_PSI_OLD, ORC_SX, ORC_SY, ORC_SZ, ORC_DHAT,
-  ORC_MU, ORC_NFIELDS
+  ORC_MU, ORC_DTPSI, ORC_RHO_OLD, ORC_NFIELDS
 };
 
 typedef struct {
@@ -457,8 +597,6 @@ void orc_destroy(orc_state *s) {
   free(s);
 }
 
-#define FOR3 for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k)
-
 static double scale_of(const orc_params *p) { return 0.125 / ((double)p->nx * p->ny * p->nz); }
 
 /* quasi.cpp:905-951 (stb 874-920): psihat = scale*DCT(psi), psi = IDCT(psihat),
@@ -474,7 +612,8 @@ void orc_set_psi(orc_state *s, const double *psi) {
   memcpy(s->f[ORC_PSIHAT_OLD], ph, s->n * sizeof(double));
   memcpy(s->f[ORC_PSI_OLD], s->f[ORC_PSI], s->n * sizeof(double));
   /* setup N (quasi.cpp:929-947) is overwritten at step 1 before it is read */
-  int zero[] = {ORC_RHOHAT, ORC_RHO, ORC_VX, ORC_VY, ORC_VZ, ORC_DIVV, ORC_DHAT, ORC_P, ORC_MU};
+  int zero[] = {ORC_RHOHAT, ORC_RHO, ORC_VX, ORC_VY, ORC_VZ, ORC_DIVV, ORC_DHAT, ORC_P,
+                ORC_MU, ORC_DTPSI, ORC_RHO_OLD};
   for (unsigned t = 0; t < sizeof(zero) / sizeof(zero[0]); ++t)
     memset(s->f[zero[t]], 0, s->n * sizeof(double));
   double *nh = s->f[ORC_NHAT];
@@ -502,6 +641,38 @@ static void fwd3(const orc_params *p, int axis_dst, double *a) {
   r2r3d_inplace(p->nx, p->ny, p->nz, kx, ky, kz, a);
 }
 
+/* S10 (quasi.cpp:1944-1965; stb 1944-1957): psihat_new from psihat, psihat_old,
+ * N and N_old; psihat_old <- psihat; psi_old <- psi; psi = IDCT psihat_new. */
+static void update(orc_state *s, const double *nhat_old) {
+  const orc_params *p = &s->p;
+  int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double *qx = s->qx, *qy = s->qy, *qz = s->qz;
+  double sc = scale_of(p), q02 = p->q0 * p->q0;
+  int stb = p->variant == 1;
+  double *ph = s->f[ORC_PSIHAT], *pho = s->f[ORC_PSIHAT_OLD];
+  const double *nh = s->f[ORC_NHAT];
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) {
+    size_t id = IDX(i, j, k);
+    double mq2 = pow(qx[i], 2) + pow(qy[j], 2) + pow(qz[k], 2);
+    double alin = p->eps - p->alpha * pow(q02 - mq2, 2);
+    double old = ph[id];
+    if (stb) {
+      /* quasi-stb.cpp:855-856, 1954-1956: Crank-Nicolson, no psihat_old term */
+      double c1 = 1.0 + (p->dt / 2) * alin, c2 = 1.0 - (p->dt / 2) * alin;
+      ph[id] = (c1 * old + (p->dt / 2) * sc * (3.0 * nh[id] - nhat_old[id])) / c2;
+    } else {
+      /* quasi.cpp:883-884, 1949-1951 */
+      double c1 = 2.0 - p->dt * p->rho_m * alin, c2 = 1.5 - p->dt * p->rho_m * alin;
+      ph[id] = (c1 * old - 0.5 * pho[id] + (p->dt / 2) * sc * (3.0 * nh[id] - nhat_old[id])) / c2;
+    }
+    pho[id] = old;
+  }
+  memcpy(s->f[ORC_PSI_OLD], s->f[ORC_PSI], s->n * sizeof(double));
+  memcpy(s->f[ORC_PSI], ph, s->n * sizeof(double));
+  inv3(p, -1, s->f[ORC_PSI]);
+}
+
 /* One step of quasi.cpp:1017-1965. */
 static void quasi_step(orc_state *s) {
   const orc_params *p = &s->p;
@@ -525,7 +696,8 @@ static void quasi_step(orc_state *s) {
   inv3(p, -1, f[ORC_F]);
   inv3(p, -1, f[ORC_G]);
 
-  /* S2: gradients of psi, lagged rho, laplacians of grad psi (quasi.cpp:1060-1232) */
+  /* S2: gradients of psi, lagged rho (from the previous step's rhohat, before
+   * S3 rewrites it), laplacians of grad psi (quasi.cpp:1060-1232) */
   for (int c = 0; c < 3; ++c) {
     orc_spec_grad(nx, ny, nz, sx, sy, sz, c, f[ORC_PSIHAT], f[ORC_GX + c]);
     orc_spec_grad(nx, ny, nz, sx, sy, sz, c, f[ORC_RHOHAT], f[ORC_RX + c]);
@@ -547,50 +719,26 @@ static void quasi_step(orc_state *s) {
     inv3(p, c, f[ORC_LX + c]);
   }
 
-  /* S3: density law and filter (quasi.cpp:1237-1268) */
-  double *rho = f[ORC_RHO], *rhohat = f[ORC_RHOHAT];
-  const double *psi = f[ORC_PSI];
-#pragma omp parallel for schedule(static)
-  for (size_t t = 0; t < s->n; ++t)
-    rho[t] = p->kp * sqrt(q02 * pow(psi[t], 2) + pow(f[ORC_GX][t], 2) + pow(f[ORC_GY][t], 2) +
-                          pow(f[ORC_GZ][t], 2)) + p->rho0;
-  memcpy(rhohat, rho, s->n * sizeof(double));
-  fwd3(p, -1, rhohat);
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) {
-    size_t id = IDX(i, j, k);
-    double mq2 = pow(qx[i], 2) + pow(qy[j], 2) + pow(qz[k], 2);
-    rhohat[id] = sc * rhohat[id] * exp(-p->filter * mq2);
-  }
-  memcpy(rho, rhohat, s->n * sizeof(double));
-  inv3(p, -1, rho);
+  /* S3: density law and filter (quasi.cpp:1237-1268); rho* goes through the
+   * D field's array, rewritten below before anything reads it */
+  double *rho = f[ORC_RHO], *dv = f[ORC_DIVV];
+  double *rstar = (double *)malloc(s->n * sizeof(double));
+  orc_density_law(p, s->n, f[ORC_PSI], f[ORC_GX], f[ORC_GY], f[ORC_GZ], rstar);
+  orc_density_filter(p, rstar, f[ORC_RHOHAT], rho);
+  free(rstar);
 
-  /* S4: divergence (quasi.cpp:1273-1293).  divv holds 0 until divvSwitch. */
-  double *dv = f[ORC_DIVV], *rold = (double *)malloc(s->n * sizeof(double));
-  if (n > p->divv_switch) {
-    /* rho_old lives in f[ORC_MU] between steps */
-    memcpy(rold, f[ORC_MU], s->n * sizeof(double));
-#pragma omp parallel for schedule(static)
-    for (size_t t = 0; t < s->n; ++t)
-      dv[t] = -((rho[t] - rold[t]) / p->dt + f[ORC_VX][t] * f[ORC_RX][t] +
-                f[ORC_VY][t] * f[ORC_RY][t] + f[ORC_VZ][t] * f[ORC_RZ][t]) /
-              (0.5 * (rho[t] + rold[t]));
-  }
-  memcpy(f[ORC_MU], rho, s->n * sizeof(double)); /* rho_old <- rho */
-  free(rold);
+  /* S4: divergence (quasi.cpp:1273-1293).  D keeps its last value (0 from
+   * set-up) up to divvSwitch; rho_old <- rho afterwards. */
+  if (n > p->divv_switch)
+    orc_divergence(p, s->n, rho, f[ORC_RHO_OLD], f[ORC_VX], f[ORC_VY], f[ORC_VZ], f[ORC_RX],
+                   f[ORC_RY], f[ORC_RZ], dv);
+  memcpy(f[ORC_RHO_OLD], rho, s->n * sizeof(double));
   memcpy(f[ORC_DHAT], dv, s->n * sizeof(double));
   fwd3(p, -1, f[ORC_DHAT]);
 
   /* S5: stress (quasi.cpp:1298-1330); G <- rho G */
-#pragma omp parallel for schedule(static)
-  for (size_t t = 0; t < s->n; ++t) {
-    double g = rho[t] * f[ORC_G][t];
-    f[ORC_G][t] = g;
-    double rf = rho[t] * f[ORC_F][t];
-    f[ORC_SX][t] = g * f[ORC_GX][t] - rf * f[ORC_LX][t];
-    f[ORC_SY][t] = g * f[ORC_GY][t] - rf * f[ORC_LY][t];
-    f[ORC_SZ][t] = g * f[ORC_GZ][t] - rf * f[ORC_LZ][t];
-  }
+  orc_stress(s->n, rho, f[ORC_G], f[ORC_F], f[ORC_GX], f[ORC_GY], f[ORC_GZ], f[ORC_LX],
+             f[ORC_LY], f[ORC_LZ], f[ORC_SX], f[ORC_SY], f[ORC_SZ]);
   for (int c = 0; c < 3; ++c) fwd3(p, c, f[ORC_SX + c]);
 
   /* S6, S7: pressure and projection (quasi.cpp:1336-1884) */
@@ -604,37 +752,14 @@ static void quasi_step(orc_state *s) {
   inv3(p, -1, dv);
 
   /* S9: nonlinear term (quasi.cpp:1905-1939) */
-  double *nh = f[ORC_NHAT];
-  double rs2 = pow(p->kp * p->amp + p->rho0, 2);
-#pragma omp parallel for schedule(static)
-  for (size_t t = 0; t < s->n; ++t) {
-    double ps = psi[t];
-    double v = rho[t] * (p->eps * ps - f[ORC_F][t] + p->beta * pow(ps, 3) - p->gamma * pow(ps, 5)) -
-               f[ORC_G][t] - f[ORC_VX][t] * f[ORC_GX][t] - f[ORC_VY][t] * f[ORC_GY][t] -
-               f[ORC_VZ][t] * f[ORC_GZ][t];
-    if (n > p->divv_switch + 1)
-      v += f[ORC_P][t] * p->kp *
-           sqrt(pow(f[ORC_RX][t], 2) + pow(f[ORC_RY][t], 2) + pow(f[ORC_RZ][t], 2)) / rs2;
-    nh[t] = v;
-  }
-  fwd3(p, -1, nh);
-  if (n == 1) memcpy(nhat_old, nh, s->n * sizeof(double));
+  orc_nonlinear(p, s->n, n, rho, f[ORC_PSI], f[ORC_F], f[ORC_G], f[ORC_VX], f[ORC_VY],
+                f[ORC_VZ], f[ORC_GX], f[ORC_GY], f[ORC_GZ], f[ORC_P], f[ORC_RX], f[ORC_RY],
+                f[ORC_RZ], f[ORC_NHAT]);
+  fwd3(p, -1, f[ORC_NHAT]);
+  if (n == 1) memcpy(nhat_old, f[ORC_NHAT], s->n * sizeof(double));
 
   /* S10: update (quasi.cpp:1944-1965) */
-  double *ph = f[ORC_PSIHAT], *pho = f[ORC_PSIHAT_OLD];
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) {
-    size_t id = IDX(i, j, k);
-    double mq2 = pow(qx[i], 2) + pow(qy[j], 2) + pow(qz[k], 2);
-    double alin = p->eps - p->alpha * pow(q02 - mq2, 2);
-    double c1 = 2.0 - p->dt * p->rho_m * alin, c2 = 1.5 - p->dt * p->rho_m * alin;
-    double old = ph[id];
-    ph[id] = (c1 * old - 0.5 * pho[id] + (p->dt / 2) * sc * (3.0 * nh[id] - nhat_old[id])) / c2;
-    pho[id] = old;
-  }
-  memcpy(f[ORC_PSI_OLD], f[ORC_PSI], s->n * sizeof(double));
-  memcpy(f[ORC_PSI], ph, s->n * sizeof(double));
-  inv3(p, -1, f[ORC_PSI]);
+  update(s, nhat_old);
   free(nhat_old);
 }
 
@@ -645,7 +770,7 @@ static void stb_step(orc_state *s) {
   int nx = p->nx, ny = p->ny, nz = p->nz;
   const double *qx = s->qx, *qy = s->qy, *qz = s->qz, *sx = s->sx, *sy = s->sy, *sz = s->sz;
   double **f = s->f;
-  double sc = scale_of(p), q02 = p->q0 * p->q0;
+  double q02 = p->q0 * p->q0;
   int n = ++s->step;
   double *nhat_old = (double *)malloc(s->n * sizeof(double));
   memcpy(nhat_old, f[ORC_NHAT], s->n * sizeof(double));
@@ -665,17 +790,9 @@ static void stb_step(orc_state *s) {
     inv3(p, c, f[ORC_GX + c]);
   }
   /* rho = 1 (quasi-stb.cpp:1232); mu and stress (1284-1308) */
-  const double *psi = f[ORC_PSI];
-  double *rho = f[ORC_RHO];
-#pragma omp parallel for schedule(static)
-  for (size_t t = 0; t < s->n; ++t) {
-    rho[t] = 1;
-    double mu = sh[t] - p->beta * pow(psi[t], 3) + p->gamma * pow(psi[t], 5);
-    f[ORC_MU][t] = mu;
-    f[ORC_SX][t] = rho[t] * mu * f[ORC_GX][t];
-    f[ORC_SY][t] = rho[t] * mu * f[ORC_GY][t];
-    f[ORC_SZ][t] = rho[t] * mu * f[ORC_GZ][t];
-  }
+  for (size_t t = 0; t < s->n; ++t) f[ORC_RHO][t] = 1;
+  orc_stb_stress(p, s->n, sh, f[ORC_PSI], f[ORC_GX], f[ORC_GY], f[ORC_GZ], f[ORC_MU],
+                 f[ORC_SX], f[ORC_SY], f[ORC_SZ]);
   for (int c = 0; c < 3; ++c) fwd3(p, c, f[ORC_SX + c]);
   /* divv_local stays 0 in stb, so Dhat = 0 */
   memset(f[ORC_DHAT], 0, s->n * sizeof(double));
@@ -683,42 +800,21 @@ static void stb_step(orc_state *s) {
               f[ORC_VZ]);
   for (int c = 0; c < 3; ++c) inv3(p, c, f[ORC_VX + c]);
 
-  /* N (quasi-stb.cpp:1893-1923) */
-  double *nh = f[ORC_NHAT];
-#pragma omp parallel for schedule(static)
-  for (size_t t = 0; t < s->n; ++t) {
-    double v = p->beta * pow(psi[t], 3) - p->gamma * pow(psi[t], 5);
-    if (p->hydro == 1)
-      v -= f[ORC_VX][t] * f[ORC_GX][t] + f[ORC_VY][t] * f[ORC_GY][t] + f[ORC_VZ][t] * f[ORC_GZ][t];
-    nh[t] = v;
-  }
-  fwd3(p, -1, nh);
-  if (n == 1) memcpy(nhat_old, nh, s->n * sizeof(double));
+  /* N and dtpsi (quasi-stb.cpp:1893-1923) */
+  orc_stb_nonlinear(p, s->n, f[ORC_PSI], f[ORC_MU], f[ORC_VX], f[ORC_VY], f[ORC_VZ], f[ORC_GX],
+                    f[ORC_GY], f[ORC_GZ], f[ORC_NHAT], f[ORC_DTPSI]);
+  fwd3(p, -1, f[ORC_NHAT]);
+  if (n == 1) memcpy(nhat_old, f[ORC_NHAT], s->n * sizeof(double));
 
   /* CN update (quasi-stb.cpp:1944-1957) */
-  double *ph = f[ORC_PSIHAT], *pho = f[ORC_PSIHAT_OLD];
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nx; ++i) for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) {
-    size_t id = IDX(i, j, k);
-    double mq2 = pow(qx[i], 2) + pow(qy[j], 2) + pow(qz[k], 2);
-    double alin = p->eps - p->alpha * pow(q02 - mq2, 2);
-    double c1 = 1.0 + (p->dt / 2) * alin, c2 = 1.0 - (p->dt / 2) * alin;
-    double old = ph[id];
-    ph[id] = (c1 * old + (p->dt / 2) * sc * (3.0 * nh[id] - nhat_old[id])) / c2;
-    pho[id] = old;
-  }
-  memcpy(f[ORC_PSI_OLD], f[ORC_PSI], s->n * sizeof(double));
-  memcpy(f[ORC_PSI], ph, s->n * sizeof(double));
-  inv3(p, -1, f[ORC_PSI]);
+  update(s, nhat_old);
 
   /* growth rate at mode (i*, 0, k*) (quasi-stb.cpp:2147-2158) */
-  if (n > 1) {
-    int is = (int)lround(p->growth_q / qx[1]);
-    int ks = (int)lround(p->q0 / qz[1]);
-    if (is < nx && ks < nz) {
-      size_t id = IDX(is, 0, ks);
-      s->growth = (ph[id] - pho[id]) / (p->dt * pho[id]);
-    }
+  int is, ks;
+  if (n > 1 && orc_stb_growth_mode(p, &is, &ks) == 0) {
+    size_t id = IDX(is, 0, ks);
+    const double *ph = f[ORC_PSIHAT], *pho = f[ORC_PSIHAT_OLD];
+    s->growth = (ph[id] - pho[id]) / (p->dt * pho[id]);
   }
   free(nhat_old);
 }
