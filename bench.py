@@ -3,29 +3,35 @@
 
 Workload (metric: grid-point updates/s at 512^3, mixed vs FP64 on one GPU):
   tilted512 -- 512^3 grid, smectic slab tilted 15 degrees about x with an
-  undulated interface, seeded noise 1e-4; quasi step (24 r2r transforms per
-  step), 1000 steps by default: the divergence and pressure terms switch on
-  after step 200 (divv_switch), so the timed steps cover both regimes in the
-  workload's proportions.  The mixed-precision path is the headline `value`;
-  the FP64 path runs beside it on the same input, and the relative L2
-  differences of psi, rho and u (as a vector) and the relative free-energy
-  and mass differences between the two at the end are reported
-  (`accuracy_mixed_vs_fp64`).
+  undulated interface, seeded noise 1e-4; quasi step, 1000 steps in the
+  workload.  The divergence D switches on after step 200 (divv_switch) and
+  the pressure term after step 201, so 20% of the workload's steps run the
+  early regime (18 transforms) and 80% the late one (23 transforms).  The
+  mixed-precision path is the headline `value`; the FP64 path runs beside it
+  on the same input, and the relative L2 differences of psi, rho and u (as a
+  vector) and the relative free-energy and mass differences between the two
+  at the end are reported (`accuracy_mixed_vs_fp64`).
 
 A step is one pass of the hot path (quasi.cpp:1017-1965) over the whole grid.
-Inputs are resident in HBM before timing starts; timing is wall clock around
-exactly `--steps` steps, with a device synchronisation on both sides (the
-steps before and after the switch are timed as two back-to-back segments,
-whose sum is the reported time).
+Inputs are resident in HBM before timing starts.  Exactly `--steps` steps
+are timed with wall clock and a device synchronisation on both sides, in two
+windows that keep the workload's proportions for any step count: round(0.2
+K) steps of the early regime right after the warmup, then (after an untimed
+advance to step divv_switch + 1) the remaining steps in the late regime.
+`value` = grid points x K / (time of both windows); `regimes` gives each
+window's ms/step and `value_workload_weighted` their 200/800 mix.
 
 Also reported:
   roofline      the pass kernel with the largest share of device time (chosen
                 over the warmup steps, which time every launch): algorithmic
-                HBM bytes of its launches in the timed region / their HIP-event
-                time (events recorded on the stream the passes run on, around
-                that kernel's launches only)
-  passes_GBs    algorithmic bytes / device time of every pass launch of the
-                warmup steps (of the timed steps when --warmup is 0)
+                HBM bytes of its launches in the timed windows / their
+                HIP-event time (events recorded on the stream the passes run
+                on, around that kernel's launches only)
+  step_traffic  per regime, the bytes one step moves: algorithmic (sum over
+                the step's launches of the fields each reads and writes) and
+                measured (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of each
+                kernel, profiles/traffic.json, times its launches per step),
+                each over the window's ms/step and as a fraction of 8 TB/s
   cpu_baseline  the CPU oracle (oracle/, FP64 C restatement of the same step)
                 on this host's cores, on a bounded 64^3 sample
 """
@@ -41,14 +47,13 @@ sys.path.insert(0, os.path.join(ROOT, "smaiso-quasi_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per grid-point update of the quasi step (24 transforms,
-# every pointwise stage fused into a transform pass): 194 field sweeps
-SWEEPS_PER_STEP = 194
+# tilted512 runs 1000 steps with divv_switch = 200: the share of early steps
+EARLY_SHARE = 0.2
 
 LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "lapgrad_x",
             "lapgrad_y", "lapgrad_z", "sh", "rhostar", "divv",
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
-            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3"]
+            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3", "stress3_divv"]
 # timer categories 40 + g of the x-pass group kernels (one kernel, several outputs)
 GROUPS = ["psi", "rho", "proj", "stb_psi", "vel"]
 
@@ -65,19 +70,74 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(symbol):
-    """HBM bytes per launch of `symbol` measured with rocprofv3 counters
+def traffic_table():
+    """{kernel symbol: HBM bytes per launch} measured with rocprofv3 counters
     (FETCH_SIZE + WRITE_SIZE, each calibrated on a copy kernel of known size)
-    and recorded in profiles/traffic.json; None when that kernel was not
-    measured."""
+    and recorded in profiles/traffic.json (empty when absent)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
-    k = t.get("kernels", {}).get(symbol)
-    return None if k is None else float(k["bytes_per_launch"])
+        return {}
+    return {k: float(v["bytes_per_launch"]) for k, v in t.get("kernels", {}).items()}
+
+
+def measured_traffic(symbol):
+    """Measured HBM bytes per launch of `symbol`, or None when that kernel was
+    not measured."""
+    return traffic_table().get(symbol)
+
+
+def kernel_symbol(k, shape, fp32):
+    """The kernel's name as rocprofv3 reports it, from the timer's fields."""
+    t = "float" if fp32 else "double"
+    n_ax = shape[k["axis"]]
+    if k["ld"] >= 40:  # x-pass group kernel
+        return f"void qsm::k_group<{t}, {n_ax}, {k['ld'] - 40}>(qsm::GroupArgs)"
+    if k["ld"] < len(LOAD_OPS) and LOAD_OPS[k["ld"]] in ("stress3", "stress_stb3", "stress3_divv"):
+        return f"void qsm::k_multi<{t}, {n_ax}, {k['ld']}>(qsm::MultiArgs)"
+    return (f"void qsm::k_pass<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+            f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
+
+
+def kernel_table(sim, steps):
+    """Every kernel timed since the last reset: symbol, launches, device ms
+    and algorithmic bytes, per step."""
+    out = []
+    rank = 0
+    while True:
+        try:
+            k = sim.timing_kernel(rank)
+        except Exception:  # past the last timed kernel
+            break
+        rank += 1
+        out.append({"symbol": kernel_symbol(k, sim.shape, sim.prec == 1),
+                    "launches": k["launches"] / steps, "ms": k["ms"] / steps,
+                    "alg_bytes": k["bytes"] / steps})
+    return out
+
+
+def step_traffic(table, ms_per_step, measured):
+    """Bytes one step moves, algorithmic and measured (falling back to the
+    algorithmic bytes of kernels without a counter measurement), over the
+    step's wall time."""
+    alg = sum(k["alg_bytes"] for k in table)
+    meas = 0.0
+    missing = []
+    for k in table:
+        b = measured.get(k["symbol"])
+        if b is None:
+            missing.append(k["symbol"])
+            meas += k["alg_bytes"]
+        else:
+            meas += b * k["launches"]
+    gbs = lambda b: b / (ms_per_step * 1e-3) / 1e9
+    return {"alg_bytes_per_step": alg, "alg_GBs": gbs(alg), "alg_frac": gbs(alg) / HBM_PEAK_GBS,
+            "measured_bytes_per_step": meas, "measured_GBs": gbs(meas),
+            "measured_frac": gbs(meas) / HBM_PEAK_GBS,
+            "kernels_without_counters": missing,
+            "kernel_ms_per_step": sum(k["ms"] for k in table)}
 
 
 def make_sim(qsm, ics, name, prec, shape):
@@ -87,19 +147,90 @@ def make_sim(qsm, ics, name, prec, shape):
     return qsm.Sim(p, prec)
 
 
-def time_steps(sim, steps, first, switch):
-    """Wall time of `steps` steps starting at step `first`, in two segments
-    split at the divergence switch: returns (total, pre, n_pre, post, n_post)."""
-    n_pre = max(0, min(steps, switch - first + 1))
+def timed(sim, steps):
+    """Wall time of `steps` steps, with a device synchronisation on both sides."""
     sim.step(0)  # synchronises the stream
     t0 = time.perf_counter()
-    if n_pre:
-        sim.step(n_pre)  # returns after a stream synchronisation
-    t1 = time.perf_counter()
-    if steps - n_pre:
-        sim.step(steps - n_pre)
-    t2 = time.perf_counter()
-    return t2 - t0, t1 - t0, n_pre, t2 - t1, steps - n_pre
+    if steps:
+        sim.step(steps)  # returns after a stream synchronisation
+    return time.perf_counter() - t0
+
+
+def run_precision(qsm, ics, name, prec, shape, psi0, args):
+    """One precision's run: warmup (every launch timed), an early-regime
+    window, an untimed advance past the switches, a late-regime window, one
+    more step with every launch timed.  Returns times, kernel tables, the
+    dominant kernel's event timing and the final fields."""
+    tag = "mixed" if prec == qsm.MIXED else "fp64"
+    log(f"{tag}: create + setup")
+    sim = make_sim(qsm, ics, name, prec, shape)
+    sim.autotune()
+    sim.set_psi(psi0)
+    switch = sim.params.divv_switch
+    k_pre = min(int(round(EARLY_SHARE * args.steps)), max(0, switch - args.warmup))
+    k_post = args.steps - k_pre
+    # warmup: every launch timed (eager); gives the early-regime kernel table
+    # and the kernel with the largest device time, whose launches alone are
+    # then timed with HIP events through the timed windows (an event pair
+    # per launch of every kernel would add ~0.4 ms per step)
+    # (with --warmup 0 nothing runs before the first window: the windows then
+    # time no kernel, and the dominant kernel and its event timing come from
+    # the one fully timed late step at the end)
+    early, sel = [], None
+    if args.warmup:
+        sim.timing(True)
+        sim.step(args.warmup)
+        early = kernel_table(sim, args.warmup)
+        top = sim.timing_kernel(0)
+        sel = (top["axis"], top["inv"], top["ld"], top["st"], top["variant"])
+    log(f"{tag}: timing {k_pre} early + {k_post} late steps")
+    if sel:
+        sim.timing_select(*sel)
+    else:
+        sim.timing(False)
+    t_pre = timed(sim, k_pre)
+    k_top = [sim.timing_kernel(0)] if sel and k_pre else []
+    # untimed advance (graph replay) to step divv_switch + 1: every later
+    # step computes D and the pressure term
+    sim.timing(False)
+    done = args.warmup + k_pre
+    if done < switch + 1:
+        sim.step(switch + 1 - done)
+        done = switch + 1
+    if sel:
+        sim.timing_select(*sel)
+    t_post = timed(sim, k_post)
+    if sel and k_post:
+        k_top.append(sim.timing_kernel(0))
+    done += k_post
+    # one late step with every launch timed: the late-regime kernel table
+    sim.timing(True)
+    sim.step(1)
+    late = kernel_table(sim, 1)
+    if not k_top:
+        k_top = [sim.timing_kernel(0)]
+    done += 1
+    sim.timing(False)
+    if not early:
+        early = late
+    fields = {f: sim.field(f) for f in (("PSI", "RHO", "VX", "VY", "VZ") if not args.no_fp64
+                                        else ("PSI",))}
+    diag = sim.diag()
+    sim.close()
+    total = t_pre + t_post
+    log(f"{tag}: {total / args.steps * 1e3:.2f} ms/step")
+    ms_pre = t_pre / k_pre * 1e3 if k_pre else None
+    ms_post = t_post / k_post * 1e3 if k_post else None
+    top_acc = None
+    if k_top:
+        top_acc = dict(k_top[0])
+        for extra in k_top[1:]:
+            for key in ("ms", "launches", "bytes"):
+                top_acc[key] += extra[key]
+    return {"s": total, "diag": diag, "fields": fields, "top": top_acc, "steps_run": done,
+            "regimes": {"steps_before_switch": k_pre, "ms_per_step_before_switch": ms_pre,
+                        "steps_after_switch": k_post, "ms_per_step_after_switch": ms_post},
+            "tables": {"early": early, "late": late}}
 
 
 def ensemble_run(qsm, ics, steps):
@@ -175,113 +306,89 @@ def main():
     name = "tilted512"
     log(f"building initial condition {shape}")
     psi0 = ics.workload_psi(name, shape=shape)
+    measured = traffic_table()
 
     res = {}
-    fields = {}
-    top = None
-    for prec, tag in ((qsm.MIXED, "mixed"), (qsm.FP64, "fp64")):
+    for prec in (qsm.MIXED, qsm.FP64):
         if prec == qsm.FP64 and args.no_fp64:
             continue
-        log(f"{tag}: create + setup")
-        sim = make_sim(qsm, ics, name, prec, shape)
-        sim.autotune()
-        sim.set_psi(psi0)
-        # the warmup steps are timed per launch (every kernel): they give the
-        # aggregate pass bandwidth and the kernel with the largest device
-        # time, whose launches alone are then timed with HIP events through
-        # the timed steps (an event pair per launch of every kernel would
-        # add ~0.4 ms per step to the timed region)
-        sim.timing(True)
-        sim.step(args.warmup)
-        kt = sim.timing_get(0)
-        if args.warmup > 0:
-            w = sim.timing_kernel(0)
-            sim.timing_select(w["axis"], w["inv"], w["ld"], w["st"], w["variant"])
-        else:
-            sim.timing(True)
-        log(f"{tag}: timing {args.steps} steps")
-        dt, t_pre, n_pre, t_post, n_post = time_steps(sim, args.steps, args.warmup + 1,
-                                                      sim.params.divv_switch)
-        if args.warmup == 0:
-            kt = sim.timing_get(0)
-        if prec == qsm.MIXED:
-            top = sim.timing_kernel(0)
-        sim.timing(False)
-        # the fields compared between the precisions (FP64 host copies; only
-        # when both runs happen, since each 512^3 copy is 1 GiB)
-        fields[tag] = {f: sim.field(f) for f in (("PSI", "RHO", "VX", "VY", "VZ") if not args.no_fp64
-                                                 else ("PSI",))}
-        diag = sim.diag()
-        sim.close()
-        res[tag] = {"s": dt, "kt": kt, "diag": diag,
-                    "regimes": {"steps_before_switch": n_pre,
-                                "ms_per_step_before_switch": t_pre / n_pre * 1e3 if n_pre else None,
-                                "steps_after_switch": n_post,
-                                "ms_per_step_after_switch": t_post / n_post * 1e3 if n_post else None}}
-        log(f"{tag}: {dt / args.steps * 1e3:.2f} ms/step")
+        res["mixed" if prec == qsm.MIXED else "fp64"] = run_precision(qsm, ics, name, prec, shape,
+                                                                       psi0, args)
 
     pts = float(np.prod(shape))
+
+    def summary(r):
+        g = r["regimes"]
+        out = {"value": pts * args.steps / r["s"], "ms_per_step": r["s"] / args.steps * 1e3,
+               "regimes": g}
+        if g["ms_per_step_before_switch"] and g["ms_per_step_after_switch"]:
+            w = EARLY_SHARE * g["ms_per_step_before_switch"] + (1 - EARLY_SHARE) * g["ms_per_step_after_switch"]
+            out["ms_per_step_workload_weighted"] = w
+            out["value_workload_weighted"] = pts / (w * 1e-3)
+        out["step_traffic"] = {}
+        for reg, key in (("early", "ms_per_step_before_switch"), ("late", "ms_per_step_after_switch")):
+            if g[key]:
+                out["step_traffic"][reg] = step_traffic(r["tables"][reg], g[key], measured)
+        return out
+
     m = res["mixed"]
-    value = pts * args.steps / m["s"]
-    k = top
-    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
-    n_ax = shape[k["axis"]]
-    if k["ld"] >= 40:  # x-pass group kernel
-        symbol = f"void qsm::k_group<float, {n_ax}, {k['ld'] - 40}>(qsm::GroupArgs)"
-    elif LOAD_OPS[k["ld"]] in ("stress3", "stress_stb3"):  # multi-output z pass
-        symbol = f"void qsm::k_multi<float, {n_ax}, {k['ld']}>(qsm::MultiArgs)"
-    else:
-        symbol = (f"void qsm::k_pass<float, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
-                  f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
+    ms = summary(m)
+    k = m["top"]
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k and k["ms"] > 0 else 0.0
+    symbol = kernel_symbol(k, shape, True)
     direction = "inverse" if k["inv"] else "forward"
     out = {
         "metric": "grid-pt updates/s at 512^3, mixed vs FP64 on 1 GPU; % HBM roofline; psi rel-L2",
-        "value": value,
+        "value": ms["value"],
         "unit": "grid-pt updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": m["s"] / args.steps * 1e3,
+        "ms_per_step": ms["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (tilted-slab initial condition, seeded noise)",
         "config": {"workload": f"{name}: quasi step, {shape[0]}x{shape[1]}x{shape[2]}, mixed precision "
-                               "(FP32 fields/transforms, FP64 psihat, coefficients, mean modes)",
+                               "(FP32 fields/transforms, FP64 psihat, coefficients, mean modes); "
+                               "timed steps split 20/80 between the regimes before and after "
+                               "the divergence switch, as in the 1000-step workload",
                    "grid": list(shape), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(symbol),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured.get(symbol),
                      "kernel": symbol,
                      "pass": f"{direction} {'xyz'[k['axis']]} pass, load op {load_op_name(k['ld'])}, "
                              f"store op {STORE_OPS[k['st']]}",
                      "avg_launch_ms": k["ms"] / max(1, k["launches"]),
                      "launches": k["launches"],
                      "bytes_per_launch": k["bytes"] / max(1, k["launches"])},
-        "regimes": m["regimes"],
-        "passes_GBs": m["kt"]["bytes"] / (m["kt"]["ms"] * 1e-3) / 1e9 if m["kt"]["ms"] > 0 else 0.0,
-        "step_roofline": {"bytes_per_update": SWEEPS_PER_STEP * 4,
-                          "achieved_GBs": value * SWEEPS_PER_STEP * 4 / 1e9,
-                          "frac": value * SWEEPS_PER_STEP * 4 / 1e9 / HBM_PEAK_GBS},
+        "regimes": ms["regimes"],
+        "value_workload_weighted": ms.get("value_workload_weighted"),
+        "ms_per_step_workload_weighted": ms.get("ms_per_step_workload_weighted"),
+        "step_traffic": ms["step_traffic"],
+        "steps_run": m["steps_run"],
     }
     if "fp64" in res:
         f = res["fp64"]
-        v64 = pts * args.steps / f["s"]
-        a, b = fields["mixed"], fields["fp64"]
-        out["fp64"] = {"value": v64, "ms_per_step": f["s"] / args.steps * 1e3, "regimes": f["regimes"],
-                       "step_roofline_frac": v64 * SWEEPS_PER_STEP * 8 / 1e9 / HBM_PEAK_GBS}
-        out["mixed_over_fp64"] = value / v64
-        def rel(fs):
-            num = sum(float(np.sum((a[f] - b[f]) ** 2)) for f in fs)
-            return float(np.sqrt(num / sum(float(np.sum(b[f] ** 2)) for f in fs)))
+        fs = summary(f)
+        out["fp64"] = {"value": fs["value"], "ms_per_step": fs["ms_per_step"], "regimes": fs["regimes"],
+                       "value_workload_weighted": fs.get("value_workload_weighted"),
+                       "step_traffic": fs["step_traffic"]}
+        out["mixed_over_fp64"] = ms["value"] / fs["value"]
+        a, b = m["fields"], f["fields"]
+
+        def rel(names):
+            num = sum(float(np.sum((a[n] - b[n]) ** 2)) for n in names)
+            return float(np.sqrt(num / sum(float(np.sum(b[n] ** 2)) for n in names)))
         out["psi_rel_l2_mixed_vs_fp64"] = rel(("PSI",))
-        dm, df = res["mixed"]["diag"], f["diag"]
+        dm, df = m["diag"], f["diag"]
         out["accuracy_mixed_vs_fp64"] = {
             "psi_rel_l2": rel(("PSI",)), "rho_rel_l2": rel(("RHO",)),
             "u_rel_l2": rel(("VX", "VY", "VZ")),  # velocity as a vector
             "energy_rel": abs(dm["energy"] - df["energy"]) / abs(df["energy"]),
             "mass_rel": abs(dm["mass"] - df["mass"]) / abs(df["mass"])}
-        out["steps_compared"] = args.warmup + args.steps
+        out["steps_compared"] = m["steps_run"]
     if not args.no_ensemble:
         log("ensemble")
         out["ensemble"] = ensemble_run(qsm, ics, args.ens_steps)

@@ -113,6 +113,8 @@ typedef struct {
 #define QSM_SY 24
 #define QSM_SZ 25
 #define QSM_DHAT 26
+#define QSM_DTPSI 27 /* stb only: dtpsi = -rho mu - v.g of the last step, the field
+                        quasi-stb.cpp:1904-1913 computes for dtpsi.dat; formed on request */
 
 typedef struct qsm_plan_s* qsm_plan;
 typedef struct qsm_handle_s* qsm_handle;

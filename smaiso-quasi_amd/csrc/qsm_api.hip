@@ -732,6 +732,18 @@ int pressure_output(qsm_handle_s* h) {
   return rc;
 }
 
+// stb: dtpsi = -rho mu - v.g (hydro = 1) of the last step, rho = 1, mu =
+// sh - beta psi^3 + gamma psi^5 (quasi-stb.cpp:1904-1913), formed on request
+// from the fields the step leaves: sh in F, the step's psi in the psi_old
+// buffer, v and grad psi.  The result goes to the transform scratch.
+int dtpsi_output(qsm_handle_s* h) {
+  const void* in[8] = {F(h, F_F), F(h, h->psi_old), F(h, F_VX), F(h, F_VY), F(h, F_VZ),
+                       F(h, F_GX), F(h, F_GY), F(h, F_GZ)};
+  hipError_t e = launch_dtpsi(h->fp32, in, (long)h->n, h->phys, F(h, F_NTMP), h->stream);
+  if (e != hipSuccess) return fail(QSM_ERR_HIP, "dtpsi launch: %s", hipGetErrorString(e));
+  return QSM_OK;
+}
+
 int stb_step(qsm_handle_s* h) {
   const long long n = ++h->step;
   const bool mixed = h->fp32;
@@ -1008,12 +1020,16 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
 namespace {
 
 // Everything a step's kernel arguments depend on: the buffers behind the
-// ping-pong fields and the phase flags of step n = h->step + 1.
+// ping-pong fields and the phase flags of step n = h->step + 1.  The stb step
+// reads no divergence or pressure (quasi-stb.cpp:1273-1282 are commented
+// out), so its key leaves out the divv_switch flags: one signature serves
+// every step after the first.
 std::vector<long long> step_key(const qsm_handle_s* h) {
   const long long n = h->step + 1;
   const qsm_params& p = h->p;
+  const bool quasi = p.variant == QSM_QUASI;
   return {(long long)h->psi_cur, (long long)h->rho_cur, (long long)(uintptr_t)h->psihat,
-          n == 1, n >= 2, n > p.divv_switch, n > p.divv_switch + 1};
+          n == 1, n >= 2, quasi && n > p.divv_switch, quasi && n > p.divv_switch + 1};
 }
 
 int one_step(qsm_handle_s* h) {
@@ -1164,6 +1180,15 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
     case QSM_NHAT: src = F(h, F_NHAT_OLD); break;
     case QSM_SX: case QSM_SY: case QSM_SZ: src = F(h, F_SX + (which - QSM_SX)); break;
     case QSM_DHAT: src = F(h, F_DHAT); break;
+    case QSM_DTPSI: {
+      if (h->p.variant != QSM_STB)
+        return fail(QSM_ERR_ARG, "qsm_get_field: dtpsi is an output of the stb variant only");
+      if (h->step < 1) return fail(QSM_ERR_STATE, "qsm_get_field: dtpsi exists after a step");
+      const int rc = dtpsi_output(h);
+      if (rc) return rc;
+      src = F(h, F_NTMP);
+      break;
+    }
     default: return fail(QSM_ERR_ARG, "qsm_get_field: unknown field %d", which);
   }
   HIPCHK(hipStreamSynchronize(h->stream));

@@ -110,6 +110,10 @@ hipError_t launch_diag(int prec_fp32, const void* psi, const void* pold, const v
 hipError_t launch_finish(const double* part, long nparts, double* out, hipStream_t s);
 // dst[i] = (float)src[i], i < n (the FP32 copy of psihat in mixed precision)
 hipError_t launch_round32(const double* src, float* dst, long n, hipStream_t s);
+// stb dtpsi = -(sh - beta psi^3 + gamma psi^5) [- v.g if hydro == 1]
+// (quasi-stb.cpp:1904-1913) over n elements; in: sh, psi, vx, vy, vz, gx, gy, gz
+hipError_t launch_dtpsi(int prec_fp32, const void* const* in, long n, const Phys& P, void* out,
+                        hipStream_t s);
 // count independent compensated sums: out[q][0] = sum of parts[q][0 .. n[q])
 hipError_t launch_finish_batch(const double* const* parts, const long* n, int count,
                                double* const* out, hipStream_t s);

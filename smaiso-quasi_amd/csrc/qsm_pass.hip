@@ -241,6 +241,40 @@ hipError_t launch_finish(const double* part, long nparts, double* out, hipStream
   return hipGetLastError();
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_dtpsi(const T* __restrict__ sh, const T* __restrict__ psi, const T* __restrict__ vx,
+            const T* __restrict__ vy, const T* __restrict__ vz, const T* __restrict__ gx,
+            const T* __restrict__ gy, const T* __restrict__ gz, long n, Phys P, T* __restrict__ out) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ps = psi[i];
+    const double mu = (double)sh[i] - P.beta * (ps * ps * ps) + P.gamma * (ps * ps * ps * ps * ps);
+    double d = -1.0 * mu;  // rho = 1 in the pure smectic
+    if (P.hydro == 1)
+      d -= (double)vx[i] * gx[i] + (double)vy[i] * gy[i] + (double)vz[i] * gz[i];
+    out[i] = (T)d;
+  }
+}
+
+hipError_t launch_dtpsi(int prec_fp32, const void* const* in, long n, const Phys& P, void* out,
+                        hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long nb = (n + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  if (prec_fp32)
+    hipLaunchKernelGGL(k_dtpsi<float>, dim3((unsigned)nb), dim3(256), 0, s, (const float*)in[0],
+                       (const float*)in[1], (const float*)in[2], (const float*)in[3],
+                       (const float*)in[4], (const float*)in[5], (const float*)in[6],
+                       (const float*)in[7], n, P, (float*)out);
+  else
+    hipLaunchKernelGGL(k_dtpsi<double>, dim3((unsigned)nb), dim3(256), 0, s, (const double*)in[0],
+                       (const double*)in[1], (const double*)in[2], (const double*)in[3],
+                       (const double*)in[4], (const double*)in[5], (const double*)in[6],
+                       (const double*)in[7], n, P, (double*)out);
+  return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) k_round32(const double* __restrict__ src,
                                                  float* __restrict__ dst, long n) {
   const long stride = (long)gridDim.x * blockDim.x;

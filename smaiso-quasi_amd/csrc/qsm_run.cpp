@@ -18,7 +18,8 @@
 //   out=DIR                output directory (reference file names and formats:
 //                          info, L1.dat, mass.dat, energy.dat, psiMid.dat,
 //                          vx.dat, vy.dat, vz.dat, divv.dat, p.dat, rho.dat,
-//                          psi<rank>.dat; stb: growth.dat, fourier.dat)
+//                          psi<rank>.dat; stb: L1.dat, mass.dat, growth.dat,
+//                          psiMid.dat at 8 decimals, dtpsi.dat, fourier.dat)
 //   fc=SIZE                focal-conic size aE = bE (argv[3] of the reference)
 //   noise=1e-4 seed=1      Gaussian noise added to psi (std::mt19937_64)
 //   graphs=1               replay steps as HIP graphs
@@ -163,11 +164,15 @@ void append_value(const std::string& path, double v) {
   o << v << "\n";
 }
 
-// mid-plane j = Ny/2 slice in [i][k] order, appended (quasi.cpp:2040-2273)
+// mid-plane j = Ny/2 slice in [i][k] order, appended (quasi.cpp:2040-2273,
+// quasi-stb.cpp:2040-2108); fixed8: std::fixed with 8 decimals, the stb
+// psiMid.dat format (quasi-stb.cpp:2065), else the stream default (6
+// significant digits)
 void append_mid(qsm_handle h, const qsm_params& p, int which, const std::string& path,
-                std::vector<double>& buf) {
+                std::vector<double>& buf, bool fixed8 = false) {
   check(qsm_get_field(h, which, buf.data()), "get_field");
   std::ofstream o(path, std::ios_base::app);
+  if (fixed8) o << std::fixed << std::setprecision(8);
   const int j = p.ny / 2;
   for (int i = 0; i < p.nx; ++i)
     for (int k = 0; k < p.nz; ++k) o << buf[at(p, i, j, k)] << "\n";
@@ -285,13 +290,15 @@ int main(int argc, char** argv) {
         saves = 0;
         check(qsm_get_field(h, QSM_PSI, psi.data()), "get_field");
         write_psi_ranks(p, psi, out, ranks);
-        const std::pair<int, const char*> mids[] = {
-            {QSM_PSI, "psiMid.dat"}, {QSM_VX, "vx.dat"}, {QSM_VY, "vy.dat"}, {QSM_VZ, "vz.dat"},
-            {QSM_DIVV, "divv.dat"}, {QSM_P, "p.dat"}, {QSM_RHO, "rho.dat"}};
-        for (const auto& m : mids)
-          if (variant == QSM_QUASI || m.first == QSM_PSI || m.first == QSM_VX || m.first == QSM_VY ||
-              m.first == QSM_VZ)
-            append_mid(h, p, m.first, out + "/" + m.second, buf);
+        if (variant == QSM_QUASI) {  // quasi.cpp:2040-2273
+          const std::pair<int, const char*> mids[] = {
+              {QSM_PSI, "psiMid.dat"}, {QSM_VX, "vx.dat"}, {QSM_VY, "vy.dat"}, {QSM_VZ, "vz.dat"},
+              {QSM_DIVV, "divv.dat"}, {QSM_P, "p.dat"}, {QSM_RHO, "rho.dat"}};
+          for (const auto& m : mids) append_mid(h, p, m.first, out + "/" + m.second, buf);
+        } else {  // quasi-stb.cpp:2040-2108 (its v slices are commented out)
+          append_mid(h, p, QSM_PSI, out + "/psiMid.dat", buf, true);
+          append_mid(h, p, QSM_DTPSI, out + "/dtpsi.dat", buf);
+        }
         if (variant == QSM_STB) {  // spectral slice j = 0, 16 digits (quasi-stb.cpp:2127-2139)
           check(qsm_get_field(h, QSM_PSIHAT, buf.data()), "get_field");
           std::ofstream o(out + "/fourier.dat", std::ios_base::app);

@@ -27,7 +27,7 @@ QUASI, STB = 0, 1
 FIELDS = [
     "PSI", "RHO", "VX", "VY", "VZ", "P", "DIVV", "PSIHAT", "F", "G", "GX", "GY", "GZ",
     "RX", "RY", "RZ", "LX", "LY", "LZ", "RHOHAT", "NHAT", "PSIHAT_OLD", "PSI_OLD",
-    "SX", "SY", "SZ", "DHAT",
+    "SX", "SY", "SZ", "DHAT", "DTPSI",
 ]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 

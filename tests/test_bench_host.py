@@ -26,3 +26,30 @@ def test_traffic_of_dominant_kernel_is_its_field_read_and_write():
     with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
         cal = json.load(f)["calibration"]
     assert abs(cal["fetch_factor"] - 2.0) < 1e-3 and cal["write_factor"] == 1.0
+
+
+def test_kernel_symbols_match_rocprof_names():
+    sys.path.insert(0, ROOT)
+    import bench
+    shape = (512, 512, 512)
+    k = {"axis": 1, "inv": 1, "ld": 0, "st": 0, "variant": 0}
+    assert bench.kernel_symbol(k, shape, True) == DOMINANT
+    assert bench.kernel_symbol(dict(k, axis=0, ld=42), shape, False) == \
+        "void qsm::k_group<double, 512, 2>(qsm::GroupArgs)"
+    assert bench.kernel_symbol(dict(k, axis=2, inv=0, ld=26), shape, True) == \
+        "void qsm::k_multi<float, 512, 26>(qsm::MultiArgs)"
+
+
+def test_step_traffic_sums_launches_times_bytes():
+    """Measured bytes per launch times launches per step, the algorithmic
+    bytes standing in for a kernel without counters, over the step time."""
+    sys.path.insert(0, ROOT)
+    import bench
+    table = [{"symbol": "a", "launches": 3.0, "ms": 1.0, "alg_bytes": 3e9},
+             {"symbol": "b", "launches": 1.0, "ms": 0.5, "alg_bytes": 2e9}]
+    t = bench.step_traffic(table, 2.0, {"a": 1.1e9})
+    assert abs(t["measured_bytes_per_step"] - (3.3e9 + 2e9)) < 1
+    assert t["kernels_without_counters"] == ["b"]
+    assert abs(t["alg_GBs"] - 5e9 / 2e-3 / 1e9) < 1e-9
+    assert abs(t["measured_frac"] - 5.3e9 / 2e-3 / 1e9 / 8000.0) < 1e-12
+    assert t["kernel_ms_per_step"] == 1.5

@@ -17,7 +17,8 @@ properties fixed by the transform definitions and by the step itself:
   (the reference's `scale`, quasi.cpp:219);
 * linearity of the 512^3 FP64 transform on two random fields;
 * the step: mixed precision against FP64 on the same initial condition,
-  the precision contract of the mixed path (psi rel-L2 <= 1e-5).
+  the precision contract of the mixed path (psi rel-L2 <= 1e-5 after the
+  workload's 1000 steps at 512^3; 50 steps at 1024x1024x512).
 
 Tolerances: FP64 1e-12 (peak-relative max error, or rel-L2), FP32 1e-6
 peak-relative and 2e-6 rel-L2, the FP32 transform tolerance of the
@@ -156,10 +157,12 @@ def _run(qsm, name, prec, steps, psi):
     return out, dg
 
 
-@pytest.mark.parametrize("name,steps", (("tilted512", 100), ("sphere1024", 20)))
+@pytest.mark.parametrize("name,steps", (("tilted512", 1000), ("sphere1024", 50)))
 def test_step_mixed_matches_fp64(qsm_mod, name, steps):
-    """The mixed path against FP64 on the workloads the benchmark and the
-    largest configuration run (the bench reports the 1000-step figures)."""
+    """The mixed path against FP64 at the step counts of the contract: the
+    512^3 tilted slab for its 1000 steps (psi rel-L2 <= 1e-5; 800 of them past
+    the divergence switch, with D, grad rho and the pressure term live) and
+    the 1024x1024x512 sphere for the 50 steps its accuracy is quoted at."""
     import ics
     print(f"step {name} {steps} steps: initial condition", flush=True)
     psi = ics.workload_psi(name)

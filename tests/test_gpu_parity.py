@@ -235,7 +235,7 @@ def test_stb_fp64(qsm_mod, oracle_mod):
         sim.step(1)
         orc.step(1)
         _cmp_fields(sim, orc, ["GX", "GY", "GZ", "SX", "SY", "SZ", "VX", "VY", "VZ", "NHAT",
-                               "PSIHAT", "PSI"], 1e-10, f"stb step{s + 1}")
+                               "PSIHAT", "PSI", "DTPSI"], 1e-10, f"stb step{s + 1}")
     g = sim.diag()["growth"]
     assert abs(g - orc.growth) <= 1e-8 * max(1.0, abs(orc.growth)), (g, orc.growth)
 

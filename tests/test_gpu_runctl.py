@@ -1,0 +1,333 @@
+"""GPU tests of the run control and the host outputs around the step, each
+against the oracle or a closed form:
+
+* a numerical blow-up is reported as QSM_ERR_NONFINITE by qsm_get_diag and
+  qsm_run_l1, where the reference loop `while (L1 > limL1)` would stop
+  silently on NaN (quasi.cpp:1014, 2010);
+* qsm_run_l1 stops at the same step as the reference's L1 loop run on the
+  oracle (L1 checked every stepL1 steps, quasi.cpp:1969-2031);
+* interface_z (this build's diagnostic, Lz (<rho> - rho0)/(rho_s - rho0)):
+  Lz for a box full of smectic, 0 for an isotropic one, Lz/2 for the
+  interface workload (a symmetric tanh blend at Lz/2);
+* the qsm_run driver's reference-format files hold the oracle's values at
+  their printed precision (quasi.cpp:2008-2273, quasi-stb.cpp:1972-2158);
+* resumes from the reference's psi<rank>.dat text (quasi.cpp:692-717) and
+  from the binary checkpoint continue as the oracle does;
+* the 96-instance stability ensemble (500 steps): every 8th instance equals
+  the same instance run alone, the two smallest match the oracle, and the
+  mixed growth rates follow the FP64 ones.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "smaiso-quasi_amd", "qsm_run")
+
+
+def _params(qsm, d):
+    return qsm.Params(**{k: d[k] for k, _ in qsm.Params._fields_})
+
+
+def _interface(shape, **over):
+    import ics
+    d = ics.params_dict("interface64", **over)
+    d["nx"], d["ny"], d["nz"] = shape
+    if d["compat_ranks"] and d["nx"] % d["compat_ranks"]:
+        d["compat_ranks"] = 0
+    return d
+
+
+# ---------------------------------------------------------------- blow-up
+
+def test_blow_up_is_reported(qsm_mod):
+    """dt = 0.2 with rho_m = 0.2 leaves the explicit part of the step unstable
+    (the oracle overflows at step 9): qsm_get_diag then returns
+    QSM_ERR_NONFINITE, and qsm_run_l1 stops with it instead of ending the loop."""
+    import ics
+    d = _interface((32, 32, 32), dt=0.2, rho_m=0.2, compat_ranks=0)
+    psi = ics.workload_psi("interface64", shape=(32, 32, 32))
+    sim = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    sim.set_psi(psi)
+    code = None
+    for n in range(1, 101):
+        sim.step(1)
+        try:
+            sim.diag()
+        except qsm_mod.QsmError as e:
+            code = e.code
+            break
+    print(f"blow-up reported at step {n}", flush=True)
+    assert code == -4 and n < 60, (code, n)
+    sim2 = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    sim2.set_psi(psi)
+    with pytest.raises(qsm_mod.QsmError) as ei:
+        sim2.run_l1(1000, step_l1=5, lim_l1=1e-6)
+    assert ei.value.code == -4
+
+
+# ---------------------------------------------------------------- L1 stop
+
+def test_run_l1_stops_where_the_oracle_loop_stops(qsm_mod, oracle_mod):
+    """The reference loop checks L1 every stepL1 steps and stops once it is
+    <= limL1.  limL1 is placed between two of the oracle's L1 checks, far
+    from both, so the loop must stop at that check."""
+    import ics
+    shape = (32, 32, 48)
+    d = _interface(shape, compat_ranks=0)
+    psi = ics.workload_psi("interface64", shape=shape)
+    step_l1 = 5
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(psi)
+    seq = []
+    for _ in range(16):
+        orc.step(step_l1)
+        seq.append(orc.diag()["L1"])
+    k = next(i for i in range(4, 16) if seq[i] < 0.9 * min(seq[:i]))
+    lim = np.sqrt(seq[k] * min(seq[:k]))
+    print(f"oracle L1 checks {['%.4g' % v for v in seq]}; limL1 {lim:.4g} stops at check {k}",
+          flush=True)
+    sim = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    sim.set_psi(psi)
+    done = sim.run_l1(1000, step_l1=step_l1, lim_l1=lim)
+    assert done == (k + 1) * step_l1, done
+    ref = oracle_mod.State(oracle_mod.make_params(**d))
+    ref.set_psi(psi)
+    ref.step(done)
+    assert rel_l2(sim.field("PSI"), ref.field("PSI")) < 1e-10
+    assert abs(sim.diag()["L1"] - seq[k]) <= 1e-9 * seq[k]
+    # max_steps caps the loop
+    sim3 = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    sim3.set_psi(psi)
+    assert sim3.run_l1(12, step_l1=step_l1, lim_l1=lim) == 12
+
+
+# ---------------------------------------------------------------- interface_z
+
+def test_interface_position(qsm_mod):
+    import ics
+    shape = (16, 16, 64)
+    d = _interface(shape, compat_ranks=0)
+    dz = 2 * np.pi / (d["q0"] * d["nw"])
+    lz = shape[2] * dz
+    z = (np.arange(shape[2]) + 0.5) * dz
+    cases = (("smectic", np.broadcast_to(d["amp"] * np.cos(d["q0"] * z), shape), lz, 1e-10),
+             ("isotropic", np.zeros(shape), 0.0, 1e-10),
+             ("interface", ics.workload_psi("interface64", shape=shape, noise=0), lz / 2, dz))
+    for name, psi, want, tol in cases:
+        sim = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+        sim.set_psi(psi)
+        sim.step(1)
+        zi = sim.diag()["interface_z"]
+        print(f"interface_z {name}: {zi:.6f} (Lz {lz:.4f}, expected {want:.4f})", flush=True)
+        assert abs(zi - want) <= tol * (lz if tol < 1 else 1.0), (name, zi, want)
+
+
+# ---------------------------------------------------------------- driver outputs
+
+def _slice(field, j):
+    return field[:, j, :].reshape(-1)
+
+
+def _close(txt_vals, ref, rel, what):
+    ref = np.asarray(ref)
+    err = np.abs(txt_vals - ref)
+    bound = rel * np.abs(ref) + 1e-6 * rel * max(float(np.max(np.abs(ref))), 1e-300)
+    assert np.all(err <= bound), (what, float(np.max(err / np.maximum(bound, 1e-300))))
+
+
+def test_driver_outputs_hold_oracle_values(qsm_mod, oracle_mod, tmp_path):
+    """qsm_run (quasi, no noise, 4-rank halo map) writes L1/mass/energy every
+    5 steps and the mid-plane slices and psi<rank>.dat every 10: each value
+    equals the oracle's at the stream's 6 significant digits."""
+    import ics
+    shape = (32, 32, 32)
+    d = _interface(shape, divv_switch=12)
+    out = tmp_path / "run"
+    keys = ["nx=32", "ny=32", "nz=32", "eps=-0.5", "beta=2", "gamma=1", "kp=0.2", "rho0=0.9",
+            "amp=0.5", "rho_m=0.95", "dt=0.01", "nu=1", "compat_ranks=4", "divv_switch=12",
+            "noise=0", "ic=interface"]
+    r = subprocess.run([RUN] + keys + ["steps=20", "step_l1=5", "step_save=2", f"out={out}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(ics.workload_psi("interface64", shape=shape, noise=0))
+    series = {"L1.dat": [], "mass.dat": [], "energy.dat": []}
+    slices = {"psiMid.dat": [], "rho.dat": [], "vx.dat": [], "vy.dat": [], "vz.dat": [],
+              "divv.dat": [], "p.dat": []}
+    names = {"psiMid.dat": "PSI", "rho.dat": "RHO", "vx.dat": "VX", "vy.dat": "VY", "vz.dat": "VZ",
+             "divv.dat": "DIVV", "p.dat": "P"}
+    for n in range(5, 21, 5):
+        orc.step(5)
+        dg = orc.diag()
+        series["L1.dat"].append(dg["L1"])
+        series["mass.dat"].append(dg["mass"])
+        series["energy.dat"].append(dg["energy"])
+        if n % 10 == 0:
+            for f, name in names.items():
+                slices[f].append(_slice(orc.field(name), shape[1] // 2))
+    for f, ref in series.items():
+        _close(np.loadtxt(out / f), ref, 1e-5, f)
+    for f, ref in slices.items():
+        got = np.loadtxt(out / f)
+        assert got.shape == (2 * 32 * 32,), f
+        _close(got, np.concatenate(ref), 1e-5, f)
+    psi = orc.field("PSI")
+    for rk in range(4):
+        _close(np.loadtxt(out / f"psi{rk}.dat"), psi[8 * rk:8 * (rk + 1)].reshape(-1), 1e-5,
+               f"psi{rk}.dat")
+
+
+def test_stb_driver_outputs(qsm_mod, oracle_mod, tmp_path):
+    """qsm_run variant=stb writes every step (stepL1 = stepSave = 1,
+    quasi-stb.cpp:149-150): L1.dat, mass.dat, growth.dat (from step 2),
+    psiMid.dat with 8 decimals (quasi-stb.cpp:2065), dtpsi.dat
+    (quasi-stb.cpp:2081-2108) and fourier.dat with 16 (quasi-stb.cpp:2140),
+    holding the oracle's values."""
+    import ics
+    qp = qsm_mod.default_params(qsm_mod.STB, nx=32, ny=32, nz=64, eps=-0.1, nu=0.5)
+    d = qp.as_dict()
+    out = tmp_path / "stb"
+    keys = ["variant=1", "nx=32", "ny=32", "nz=64", "eps=-0.1", "nu=0.5", "ic=stb", "phi=0.1",
+            "steps=3", f"out={out}"]
+    r = subprocess.run([RUN] + keys, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(ics.stb_psi((32, 32, 64), amp=d["amp"], qi=d["growth_q"], phi=0.1))
+    L1, mass, growth, mid, dtp, four = [], [], [], [], [], []
+    for n in range(1, 4):
+        orc.step(1)
+        dg = orc.diag()
+        L1.append(dg["L1"])
+        mass.append(dg["mass"])
+        if n > 1:
+            growth.append(orc.growth)
+        mid.append(_slice(orc.field("PSI"), 16))
+        dtp.append(_slice(orc.field("DTPSI"), 16))
+        four.append(_slice(orc.field("PSIHAT"), 0))
+    _close(np.loadtxt(out / "L1.dat"), L1, 1e-5, "L1.dat")
+    _close(np.loadtxt(out / "mass.dat"), mass, 1e-5, "mass.dat")
+    _close(np.loadtxt(out / "growth.dat"), growth, 1e-5, "growth.dat")
+    lines = (out / "psiMid.dat").read_text().split()
+    assert len(lines) == 3 * 32 * 64 and all(re.fullmatch(r"-?\d+\.\d{8}", x) for x in lines)
+    assert np.max(np.abs(np.array(lines, dtype=float) - np.concatenate(mid))) <= 5.1e-9
+    _close(np.loadtxt(out / "dtpsi.dat"), np.concatenate(dtp), 1e-5, "dtpsi.dat")
+    four_txt = (out / "fourier.dat").read_text().split()
+    assert all(re.fullmatch(r"-?\d+\.\d{16}", x) for x in four_txt[:50])
+    assert np.max(np.abs(np.array(four_txt, dtype=float) - np.concatenate(four))) <= 1e-14
+    for f in ("vx.dat", "vy.dat", "vz.dat", "energy.dat"):
+        assert not (out / f).exists(), f
+
+
+# ---------------------------------------------------------------- resume
+
+def test_text_restart_matches_oracle(qsm_mod, oracle_mod, tmp_path):
+    """A run restarted from psi<rank>.dat files (4 ranks, [i_local][j][k],
+    6 digits) is a cold start from the loaded psi (quasi.cpp:692-717, 940):
+    the driver's state after 5 more steps equals the oracle's from the same
+    text."""
+    shape = (32, 32, 32)
+    d = _interface(shape)
+    a = tmp_path / "a"
+    base = ["nx=32", "ny=32", "nz=32", "eps=-0.5", "beta=2", "gamma=1", "kp=0.2", "rho0=0.9",
+            "amp=0.5", "rho_m=0.95", "dt=0.01", "nu=1", "compat_ranks=4"]
+    r = subprocess.run([RUN] + base + ["ic=interface", "steps=10", "step_l1=5", "step_save=2",
+                                       f"out={a}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ck = tmp_path / "b.ckpt"
+    r = subprocess.run([RUN] + base + ["ic=load", f"load={a}", "ranks=4", "steps=5", "step_l1=5",
+                                       f"ckpt_out={ck}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    loaded = np.concatenate([np.loadtxt(a / f"psi{rk}.dat") for rk in range(4)]).reshape(shape)
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(loaded)
+    orc.step(5)
+    sim = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    sim.checkpoint_read(str(ck))
+    assert sim.diag()["step"] == 5
+    for f in ("PSI", "RHO", "VX", "PSIHAT"):
+        e = rel_l2(sim.field(f), orc.field(f))
+        assert e < 1e-10, (f, e)
+
+
+def test_checkpoint_resume_matches_oracle(qsm_mod, oracle_mod, tmp_path):
+    """5 steps, a binary checkpoint, a new handle resumes and takes 4 more
+    across the divergence switch (divv_switch = 6): the oracle's 9-step run."""
+    import ics
+    shape = (32, 32, 48)
+    d = _interface(shape, divv_switch=6, compat_ranks=0)
+    psi = ics.workload_psi("interface64", shape=shape)
+    a = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    a.set_psi(psi)
+    a.step(5)
+    path = str(tmp_path / "s.ckpt")
+    a.checkpoint_write(path)
+    a.close()
+    b = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
+    b.checkpoint_read(path)
+    b.step(4)
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(psi)
+    orc.step(9)
+    # v, p and D carry the round-off amplification of the late regime (a
+    # 1e-16 change of psi moves v by ~1e-10 within two steps in the oracle
+    # itself), so they are held to 1e-8
+    for f, t in (("PSI", 1e-10), ("RHO", 1e-10), ("PSIHAT", 1e-10), ("NHAT", 1e-10),
+                 ("VX", 1e-8), ("VY", 1e-8), ("VZ", 1e-8), ("P", 1e-8), ("DIVV", 1e-8)):
+        e = rel_l2(b.field(f), orc.field(f))
+        print(f"resumed step 9 {f}: {e:.2e} (tol {t:.0e})", flush=True)
+        assert e < t, (f, e)
+
+
+# ---------------------------------------------------------------- 96-instance ensemble
+
+def test_ensemble_96_members_500_steps(qsm_mod, oracle_mod):
+    """The stability-ensemble workload: 96 stb instances with grid edges from
+    {32, 48, 64, 96, 128}, 500 steps, FP64 and mixed, batched passes replayed
+    from graphs.  Every 8th instance equals the same instance run alone bit
+    for bit; the two smallest instances match the oracle; mixed growth rates
+    follow FP64."""
+    import ics
+    members = ics.ensemble_members(96, seed=4)
+    params = [_params(qsm_mod, d) for d, _ in members]
+    steps = 500
+    growth = {}
+    for prec in (qsm_mod.FP64, qsm_mod.MIXED):
+        e = qsm_mod.Ensemble(params, prec)
+        for i, (_, psi) in enumerate(members):
+            e.set_psi(i, psi)
+        e.step(steps)
+        growth[prec] = e.growth()
+        assert np.all(np.isfinite(growth[prec]))
+        print(f"ensemble prec {prec}: growth[:4] {growth[prec][:4]}", flush=True)
+        for i in range(0, 96, 8):
+            s = qsm_mod.Sim(params[i], prec)
+            s.set_psi(members[i][1])
+            s.step(steps)
+            assert np.array_equal(s.field("PSI"), e.field(i, "PSI")), (prec, i)
+            assert s.diag()["growth"] == growth[prec][i], (prec, i)
+            s.close()
+        if prec == qsm_mod.FP64:
+            sizes = [d["nx"] * d["ny"] * d["nz"] for d, _ in members]
+            for i in np.argsort(sizes)[:2]:
+                orc = oracle_mod.State(oracle_mod.make_params(**members[i][0]))
+                orc.set_psi(members[i][1])
+                orc.step(steps)
+                err = rel_l2(e.field(int(i), "PSI"), orc.field("PSI"))
+                print(f"ensemble member {i} {members[i][0]['nx']}x{members[i][0]['ny']}x"
+                      f"{members[i][0]['nz']} vs oracle: psi {err:.2e}, growth "
+                      f"{growth[prec][i]:.10e} / {orc.growth:.10e}", flush=True)
+                assert err < 1e-10
+                assert abs(growth[prec][i] - orc.growth) <= 1e-8 * max(1.0, abs(orc.growth))
+        e.close()
+    g64, g32 = growth[qsm_mod.FP64], growth[qsm_mod.MIXED]
+    rel = float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)))
+    print(f"growth mixed vs fp64: max rel {rel:.2e}", flush=True)
+    assert rel < 1e-3
