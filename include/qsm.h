@@ -136,6 +136,15 @@ int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void* hip_strea
 /* same with host arrays (copies in and out; the plan's precision decides
  * the element type) */
 int qsm_r2r3d_execute_host(qsm_plan p, const void* h_in, void* h_out);
+/* The same as qsm_r2r3d_execute_host without any device allocation, for
+ * callers that cannot allocate (allocation-free harnesses, tight loops that
+ * call a transform per fftw_execute of the reference).  No counterpart in the
+ * reference.  The field is staged in a 128 MiB buffer the library holds per
+ * GPU (at most 2^24 FP64 or 2^25 FP32 points: 256^3 FP64); larger fields
+ * return QSM_ERR_UNSUPPORTED.  Calls on one GPU share that buffer: calls that
+ * overlap on one GPU (from several threads) race and must be serialised by
+ * the caller. */
+int qsm_r2r3d_execute_host_noalloc(qsm_plan p, const void* h_in, void* h_out);
 int qsm_r2r3d_destroy(qsm_plan p);
 
 /* ---- step level: replaces the body of main() around the time loop */

@@ -238,6 +238,53 @@ extern "C" int qsm_r2r3d_execute_host(qsm_plan p, const void* h_in, void* h_out)
   return rc;
 }
 
+// ---- the allocation-free form of qsm_r2r3d_execute_host.  Its staging
+// buffer is this file-scope device array (kStageBytes: one 256^3 FP64 field,
+// the reference's default grid), held by every GPU the library runs on until
+// the process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy: the array's address on the current device comes from
+// hipGetSymbolAddress, once per device, and is kept in g_stage.
+constexpr size_t kStageBytes = (size_t)1 << 27;
+__device__ __align__(256) unsigned char scratch[kStageBytes];
+
+namespace {
+std::mutex g_stage_mu;
+std::map<int, void*> g_stage;  // device ordinal -> address of `scratch` there
+
+int stage_address(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  auto it = g_stage.find(dev);
+  if (it == g_stage.end()) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(QSM_ERR_HIP, "staging buffer has no address on device %d", dev);
+    it = g_stage.emplace(dev, ptr).first;
+  }
+  *out = it->second;
+  return QSM_OK;
+}
+}  // namespace
+
+extern "C" int qsm_r2r3d_execute_host_noalloc(qsm_plan p, const void* h_in, void* h_out) {
+  if (!p || !h_in || !h_out) return fail(QSM_ERR_ARG, "qsm_r2r3d_execute_host_noalloc: null argument");
+  const size_t bytes = (size_t)p->n[0] * p->n[1] * p->n[2] * (p->fp32 ? 4 : 8);
+  if (bytes > kStageBytes)
+    return fail(QSM_ERR_UNSUPPORTED,
+                "qsm_r2r3d_execute_host_noalloc: %zu-byte field exceeds the %zu-byte staging buffer",
+                bytes, kStageBytes);
+  void* d = nullptr;
+  int rc = stage_address(&d);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(d, h_in, bytes, hipMemcpyHostToDevice));
+  rc = qsm_r2r3d_execute(p, d, d, nullptr);
+  if (rc) return rc;
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(h_out, d, bytes, hipMemcpyDeviceToHost));
+  return QSM_OK;
+}
+
 extern "C" int qsm_r2r3d_destroy(qsm_plan p) {
   delete p;
   return QSM_OK;

@@ -72,6 +72,7 @@ SIGNATURES = {
     "qsm_r2r3d_plan": ([_IP] * 7 + [ctypes.c_uint, ctypes.POINTER(_VP)], _IP),
     "qsm_r2r3d_execute": ([_VP, _VP, _VP, _VP], _IP),
     "qsm_r2r3d_execute_host": ([_VP, _VP, _VP], _IP),
+    "qsm_r2r3d_execute_host_noalloc": ([_VP, _VP, _VP], _IP),
     "qsm_r2r3d_destroy": ([_VP], _IP),
     "qsm_create": ([ctypes.POINTER(Params), _IP, ctypes.POINTER(_VP)], _IP),
     "qsm_set_psi": ([_VP, _DP], _IP),
@@ -158,6 +159,17 @@ class Plan:
         out = np.empty_like(x)
         _check(lib().qsm_r2r3d_execute_host(self.h, x.ctypes.data_as(ctypes.c_void_p),
                                             out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def noalloc(self, x):
+        """The transform through qsm_r2r3d_execute_host_noalloc (staging in
+        the library's static device buffer, no device allocation)."""
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        if x.shape != self.shape:
+            raise ValueError(f"shape {x.shape} != plan shape {self.shape}")
+        out = np.empty_like(x)
+        _check(lib().qsm_r2r3d_execute_host_noalloc(self.h, x.ctypes.data_as(ctypes.c_void_p),
+                                                    out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
     def close(self):
