@@ -262,7 +262,14 @@ int main(int argc, char** argv) {
          << "divv switch: " << p.divv_switch << "\n" << "Initial amp: " << p.amp << "\n"
          << "kp: " << p.kp << "\n" << "rho_0: " << p.rho0 << "\n"
          << "Precision: " << (prec == QSM_MIXED ? "mixed" : "fp64") << "\n";
-    for (const char* f : {"L1.dat", "mass.dat", "energy.dat"}) std::ofstream(out + "/" + f);
+    // the files a run starts empty (quasi.cpp:633-687; quasi-stb.cpp:646-690)
+    const std::vector<const char*> fresh =
+        variant == QSM_QUASI
+            ? std::vector<const char*>{"L1.dat", "mass.dat", "energy.dat", "psiMid.dat", "vx.dat",
+                                       "vy.dat", "vz.dat", "divv.dat", "rho.dat", "p.dat"}
+            : std::vector<const char*>{"growth.dat", "L1.dat", "mass.dat", "psiMid.dat",
+                                       "dtpsi.dat", "fourier.dat"};
+    for (const char* f : fresh) std::ofstream(out + "/" + f);
   }
 
   const auto t0 = std::chrono::steady_clock::now();

@@ -37,7 +37,10 @@ def _params(qsm, d):
 
 
 def _interface(shape, **over):
+    """interface64's parameters on `shape`; rho_m as the driver's text
+    "0.95" parses, so parameter sets compare equal across the two"""
     import ics
+    over.setdefault("rho_m", 0.95)
     d = ics.params_dict("interface64", **over)
     d["nx"], d["ny"], d["nz"] = shape
     if d["compat_ranks"] and d["nx"] % d["compat_ranks"]:
@@ -90,7 +93,9 @@ def test_run_l1_stops_where_the_oracle_loop_stops(qsm_mod, oracle_mod):
     for _ in range(16):
         orc.step(step_l1)
         seq.append(orc.diag()["L1"])
-    k = next(i for i in range(4, 16) if seq[i] < 0.9 * min(seq[:i]))
+    # the first check at least 1% below every earlier one (L1 falls and then
+    # rises again on this run; the loop stops at the first L1 <= limL1)
+    k = next(i for i in range(3, 16) if seq[i] < 0.99 * min(seq[:i]))
     lim = np.sqrt(seq[k] * min(seq[:k]))
     print(f"oracle L1 checks {['%.4g' % v for v in seq]}; limL1 {lim:.4g} stops at check {k}",
           flush=True)
@@ -113,13 +118,15 @@ def test_run_l1_stops_where_the_oracle_loop_stops(qsm_mod, oracle_mod):
 
 def test_interface_position(qsm_mod):
     import ics
-    shape = (16, 16, 64)
+    shape = (32, 32, 64)
     d = _interface(shape, compat_ranks=0)
     dz = 2 * np.pi / (d["q0"] * d["nw"])
     lz = shape[2] * dz
     z = (np.arange(shape[2]) + 0.5) * dz
     cases = (("smectic", np.broadcast_to(d["amp"] * np.cos(d["q0"] * z), shape), lz, 1e-10),
-             ("isotropic", np.zeros(shape), 0.0, 1e-10),
+             # psi = 0 exactly makes L1 = 0/0 (reported as non-finite): a
+             # 1e-14 remnant moves rho by ~1e-15
+             ("isotropic", 1e-14 * np.random.default_rng(2).standard_normal(shape), 0.0, 1e-10),
              ("interface", ics.workload_psi("interface64", shape=shape, noise=0), lz / 2, dz))
     for name, psi, want, tol in cases:
         sim = qsm_mod.Sim(_params(qsm_mod, d), qsm_mod.FP64)
