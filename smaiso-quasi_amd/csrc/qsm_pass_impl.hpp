@@ -27,6 +27,10 @@
 namespace qsm {
 
 constexpr int kAxX = 0, kAxY = 1, kAxZ = 2;
+// Walk elements whose loads a fused group pass issues together: 4 keeps the
+// FP32 projection groups at 119 VGPRs with the held values and no spills
+// (8: 53 VGPRs spilled).
+constexpr int kFuseChunk = 4;
 
 
 
@@ -277,16 +281,33 @@ constexpr int smem_tables() {
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<AX, LD, ST>()>()>;
 
+// Values per thread of a pass's load walk (the tile's elements over the block).
+template <typename T, int N, int AX, int V>
+constexpr int walk_count() {
+  return TileCfg<T, N, AX, V>::TK * N / TileCfg<T, N, AX, V>::BLOCK;
+}
+
 // One workgroup's work: tile `tile` of the pass described by A, in the LDS
 // sm.  stage = false skips the staging of twiddles and tables a previous
 // pass of the same kernel (same axis and direction) left in sm.  NTS: the
 // direct stores of inverse strided passes are non-temporal.
-template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM, bool NTS = false>
+// Two further modes of inverse passes serve fused group outputs:
+//   XLD >= 0  the load walk also evaluates load operator XLD (arguments
+//             *xop) at every element it reads, into held[s] for walk element
+//             s, so a second output that reads the same fields is formed
+//             from the same reads;
+//   FROMH     the tile is not loaded: held[s] (from an XLD pass of this
+//             tile) is the input at walk element s.
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM, bool NTS = false,
+          int XLD = -1, bool FROMH = false>
 __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Phys& Ph,
                                           const LoadOp& Lop, const StoreOp& Sop,
                                           const void* twp, double* part, const int dstf,
                                           const int tile, SM& sm, const bool stage = true,
-                                          const int tid = (int)threadIdx.x) {
+                                          const int tid = (int)threadIdx.x,
+                                          const LoadOp* xop = nullptr, T* held = nullptr) {
+  static_assert((XLD < 0 && !FROMH) || (INV && AX != 2), "fused outputs: strided inverse passes");
+  static_assert(XLD < 0 || load_split<LD>() == load_split<XLD>(), "fused operators load alike");
   using C = TileCfg<T, N, AX, V>;
   using PL = PlanFor<T, N / 2>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
@@ -304,8 +325,8 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   // are staged in LDS where they fit (SM::ntab), else read per element; the
   // other axes' values are fixed per thread (strided passes: one i or j per
   // tile, one column k per thread) or per row (z passes: staged per row).
-  constexpr int NT = kv_tables<AX, LD, ST>();
-  constexpr bool KV = uses_kv<LD, ST>();
+  constexpr int NT = kv_tables<AX, LD, ST>() | (XLD >= 0 ? kv_tables<AX, XLD, ST_PLAIN>() : 0);
+  constexpr bool KV = uses_kv<LD, ST>() || (XLD >= 0 && uses_kv<XLD, ST_PLAIN>());
   constexpr bool XT_LDS = NT != 0 && (SM::tab & NT) == NT;
   constexpr bool ROWT = AX == kAxZ && SM::ntab > 0;  // per-row wavenumbers staged (z)
   // slot of table bit b in sm.xt
@@ -570,16 +591,24 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
     // ---- load X in natural order (DST-III: reversed) into LDS
     {
       constexpr int PER = TK * N / BLOCK;
-      constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+      static_assert(PER == walk_count<T, N, AX, V>(), "walk length");
+      // chunk of the walk whose loads are issued together (fused: fewer, as
+      // held[] already takes walk_count registers)
+      constexpr int CHL = XLD >= 0 ? kFuseChunk : (load_heavy<LD>() ? 8 : PER);
+      constexpr int CH = FROMH ? PER : (PER < CHL ? PER : CHL);
+      static_assert(PER % CH == 0, "whole chunks");
       WalkPos<AX, N, TK, TKP, BLOCK, 0> wn;
       WalkPos<AX, N, TK, TKP, BLOCK, 2> wr;
       wn.init(tid);
       wr.init(tid);
-#pragma unroll 1
-      for (int s0 = 0; s0 < PER; s0 += CH) {
+      auto chunk = [&](const int s0) {
         T xs[CH];
-        if constexpr (load_split<LD>()) {
+        if constexpr (FROMH) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c) xs[c] = held[s0 + c];
+        } else if constexpr (load_split<LD>()) {
           Raw4<T> raw[CH];
+          Raw4<T> xraw[XLD >= 0 ? CH : 1];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
             int q, n, i, j, k;
@@ -587,6 +616,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
             raw[c] = fetch<T, LD>(Lop, G, i, j, k, idx);
+            if constexpr (XLD >= 0) xraw[c] = fetch<T, (XLD < 0 ? LD : XLD)>(*xop, G, i, j, k, idx);
           }
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
@@ -594,7 +624,16 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             Ix idx;
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-            xs[c] = finish<T, LD>(raw[c], Lop, Ph, kv_at(i, j, k, q), G, i, j, k);
+            const Kv K = kv_at(i, j, k, q);
+            xs[c] = finish<T, LD>(raw[c], Lop, Ph, K, G, i, j, k);
+            if constexpr (XLD >= 0) {
+              // held[] as a shift register: after the last chunk, held[s]
+              // is walk element s (constant indices in a rolled loop)
+              const T xv = finish<T, (XLD < 0 ? LD : XLD)>(xraw[c], *xop, Ph, K, G, i, j, k);
+#pragma unroll
+              for (int u = 0; u + 1 < PER; ++u) held[u] = held[u + 1];
+              held[PER - 1] = xv;
+            }
           }
         } else {
 #pragma unroll
@@ -605,6 +644,8 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
             if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
             else xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
+            if constexpr (XLD >= 0)
+              held[s0 + c] = load_eval<T, (XLD < 0 ? LD : XLD)>(*xop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
           }
         }
         // values outside the field stay in their own pencil and are never
@@ -617,6 +658,14 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
 #pragma unroll
           for (int c = 0; c < CH; ++c) rbuf[wn.pos(tid, s0 + c, oddn)] = xs[c];
         }
+      };
+      if constexpr (FROMH) {
+        // held[] is indexed by walk element: every index a constant
+#pragma unroll
+        for (int s0 = 0; s0 < PER; s0 += CH) chunk(s0);
+      } else {
+#pragma unroll 1
+        for (int s0 = 0; s0 < PER; s0 += CH) chunk(s0);
       }
     }
     __syncthreads();
@@ -955,6 +1004,7 @@ struct GroupOps<GRP_PSI> {
   static constexpr bool XORD = false;
   static constexpr int PARTS = 2;  // 1495 -> 1420 us (4 and 8 parts: slower)
   static constexpr int VAR64 = 1;
+  static constexpr bool FUSED = false;
 };
 template <>
 struct GroupOps<GRP_RHO> {
@@ -964,6 +1014,7 @@ struct GroupOps<GRP_RHO> {
   static constexpr bool XORD = false;
   static constexpr int PARTS = 1;  // 3 parts: 656 -> 710 us
   static constexpr int VAR64 = 1;
+  static constexpr bool FUSED = false;
 };
 template <>
 struct GroupOps<GRP_PROJ> {
@@ -973,6 +1024,11 @@ struct GroupOps<GRP_PROJ> {
   static constexpr bool XORD = true;
   static constexpr int PARTS = 1;  // 2 parts: 1478 -> 1628 us
   static constexpr int VAR64 = 0;
+  // v_x and v_z read the same rows of S-hat and D-hat (slab i and i+1 of
+  // row j, S_y of row j-1): one load walk forms both, then p, then v_y
+  // (FP32 at 512^3: 1691 -> 1542 us)
+  static constexpr bool FUSED = true;
+  using SEQ = OpList<1, 3, 0, 2>;
 };
 template <>
 struct GroupOps<GRP_STB_PSI> {
@@ -982,6 +1038,7 @@ struct GroupOps<GRP_STB_PSI> {
   static constexpr bool XORD = false;
   static constexpr int PARTS = 1;
   static constexpr int VAR64 = 0;
+  static constexpr bool FUSED = false;
 };
 template <>
 struct GroupOps<GRP_VEL> {
@@ -991,6 +1048,9 @@ struct GroupOps<GRP_VEL> {
   static constexpr bool XORD = true;
   static constexpr int PARTS = 1;
   static constexpr int VAR64 = 0;
+  // v_x and v_z from one load walk, then v_y (FP32 at 512^3: 1186 -> 1125 us)
+  static constexpr bool FUSED = true;
+  using SEQ = OpList<0, 2, 1>;
 };
 
 template <int AX, int... LDs>
@@ -1026,6 +1086,71 @@ __device__ __forceinline__ void run_group(OpList<LD, REST...>, const GroupArgs& 
     run_group<T, N, AX, INV, SM, O + 1, LO, HI>(OpList<REST...>{}, G, tile, sm);
   }
 }
+
+// Load operator of output I of a group.
+template <int I, int... LDs>
+constexpr int op_at(OpList<LDs...>) {
+  constexpr int a[] = {LDs...};
+  return a[I];
+}
+
+// Outputs of a group in the order SEQ (output indices).  The first two are
+// fused: the first output's load walk also evaluates the second's operator at
+// every element, keeping those values in registers (walk_count of them per
+// thread), so the fields both read are read once; the second output's pass
+// then starts from the registers.  The rest run one after another as in
+// run_group.
+template <typename T, int N, int AX, bool INV, class SM, class L, int... REST>
+__device__ __forceinline__ void run_rest(OpList<REST...>, const GroupArgs& G, int tile, SM& sm) {
+  (
+      [&] {
+        __syncthreads();  // the next pass rewrites the tile in LDS
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        pass_tile<T, N, AX, INV, op_at<REST>(L{}), ST_PLAIN, SM::VAR, SM, true>(
+            G.geo, G.waves, G.phys, G.out[REST].ld, G.out[REST].st, G.tw, nullptr,
+            G.out[REST].dst, tile, sm, false, tid);
+      }(),
+      ...);
+}
+
+template <typename T, int N, int AX, bool INV, class SM, class L, int OA, int OB, int... REST>
+__device__ __forceinline__ void run_fused(OpList<OA, OB, REST...>, const GroupArgs& G, int tile,
+                                          SM& sm) {
+  constexpr int LDA = op_at<OA>(L{}), LDB = op_at<OB>(L{});
+  T held[walk_count<T, N, AX, SM::VAR>()];
+  {
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    pass_tile<T, N, AX, INV, LDA, ST_PLAIN, SM::VAR, SM, true, LDB, false>(
+        G.geo, G.waves, G.phys, G.out[OA].ld, G.out[OA].st, G.tw, nullptr, G.out[OA].dst, tile,
+        sm, true, tid, &G.out[OB].ld, held);
+  }
+  __syncthreads();
+  {
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    pass_tile<T, N, AX, INV, LDB, ST_PLAIN, SM::VAR, SM, true, -1, true>(
+        G.geo, G.waves, G.phys, G.out[OB].ld, G.out[OB].st, G.tw, nullptr, G.out[OB].dst, tile,
+        sm, false, tid, nullptr, held);
+  }
+  run_rest<T, N, AX, INV, SM, L>(OpList<REST...>{}, G, tile, sm);
+}
+
+// FP32 only: in FP64 (1024-thread tiles, at most 128 VGPRs) the held values
+// spill
+template <typename T, class GO>
+constexpr bool group_fused() {
+  return sizeof(T) == 4 && GO::FUSED;
+}
+template <class GO, bool F = GO::FUSED>
+struct GroupSeq {
+  using S = OpList<>;
+};
+template <class GO>
+struct GroupSeq<GO, true> {
+  using S = typename GO::SEQ;
+};
 
 // Tile variant of group GRP in precision T: FP64 groups with VAR64 = 1 take
 // half-width tiles (128-byte row segments, two workgroups per CU instead of
@@ -1104,7 +1229,13 @@ __global__ void __launch_bounds__((TileCfg<T, N, GroupOps<GRP>::AX, group_varian
   // contiguous range of rows per XCD keeps those in the XCD's L2
   const int tile = GO::XORD ? xcd_tile(b, nb)
                             : tile_order<T, N, AX, group_variant<T, GRP>()>(G.geo, b, nb);
-  run_part<T, N, AX, GO::INV, decltype(sm), typename GO::L, CNT, PARTS, 0>(part, G, tile, sm);
+  if constexpr (group_fused<T, GO>()) {
+    static_assert(PARTS == 1, "a fused group runs each tile in one workgroup");
+    run_fused<T, N, AX, GO::INV, decltype(sm), typename GO::L>(typename GroupSeq<GO>::S{}, G, tile,
+                                                                sm);
+  } else {
+    run_part<T, N, AX, GO::INV, decltype(sm), typename GO::L, CNT, PARTS, 0>(part, G, tile, sm);
+  }
 }
 
 // Batched pass over the instances of an ensemble that share this pass's
