@@ -129,9 +129,17 @@ int qsm_default_params(int variant, qsm_params* p);
 /* ---- transform level: replaces fftw_mpi_plan_r2r_3d / fftw_execute /
  * fftw_destroy_plan (quasi.cpp:449-495, 908-1964, 2310-2319).  One kind per
  * axis, unnormalised.  prec: QSM_FP64 (double arrays) or QSM_MIXED (float
- * arrays).  flags: reserved, pass 0.  in == out is allowed. */
+ * arrays).  flags: 0, or QSM_PLAN_MEASURE (the reference plans with
+ * FFTW_MEASURE, quasi.cpp:449-477): time every tile and radix variant of
+ * each axis's pass on a scratch field at plan time and keep the fastest.
+ * Without it every axis takes variant 0.  in == out is allowed. */
+#define QSM_PLAN_MEASURE 1u
 int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, int prec, unsigned flags,
                    qsm_plan* out);
+/* the variant of one axis's pass by hand (see qsm_set_variant);
+ * QSM_ERR_UNSUPPORTED where that axis's length has no such variant.
+ * Variants 2 and 3 round differently from 0 and 1 (other radix order). */
+int qsm_r2r3d_set_variant(qsm_plan p, int axis, int variant);
 int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void* hip_stream);
 /* same with host arrays (copies in and out; the plan's precision decides
  * the element type) */
@@ -192,17 +200,25 @@ int qsm_ens_growth(qsm_ens e, double* out);
 int qsm_ens_get_field(qsm_ens e, int index, int which, double* host);
 int qsm_ens_get_diag(qsm_ens e, int index, qsm_diag* d);
 int qsm_ens_set_graphs(qsm_ens e, int enable);
+/* steps chained in one graph launch (1, or even up to 64; default 1).  Once
+ * the step sequence repeats (from step 4 on, between phase changes such as
+ * divv_switch), `steps` steps are captured once as one graph and replayed
+ * per launch; results are bit for bit those of one graph per step. */
+int qsm_ens_set_graph_steps(qsm_ens e, int steps);
 int qsm_ens_destroy(qsm_ens e);
 
-/* ---- autotuner: times the tile-width variants of the plain transform passes
- * on the handle's (or ensemble's) own grid(s), using a scratch field, in
+/* ---- autotuner: times the variants of the plain transform passes on the
+ * handle's (or ensemble's) own grid(s), using a scratch field, in
  * interleaved rounds, and keeps per axis and direction (and pencil length for
- * ensembles) the half width only where it is at least 3% faster than the
- * default.  cache_path (may be NULL): JSON file of earlier choices, read and
- * updated. */
+ * ensembles) the fastest variant that is at least 3% faster than the
+ * default, else the default.  Variants: 0 default tile and radix plan, 1
+ * half-width tiles, 2 and 3 alternative radix plans (pencil lengths 32-512;
+ * they round differently, at the level of the transform's round-off).
+ * cache_path (may be NULL): JSON file of earlier choices, read and updated. */
 int qsm_autotune(qsm_handle h, const char* cache_path);
-/* set the variant (0 default width, 1 half width) of the plain passes along
- * axis (0 x, 1 y, 2 z) in direction inv (1: DCT-III/DST-III) by hand */
+/* set the variant (0-3, as above) of the plain passes along axis (0 x, 1 y,
+ * 2 z) in direction inv (1: DCT-III/DST-III) by hand; QSM_ERR_UNSUPPORTED
+ * where that axis's length has no such variant */
 int qsm_set_variant(qsm_handle h, int axis, int inv, int variant);
 int qsm_ens_autotune(qsm_ens e, const char* cache_path);
 

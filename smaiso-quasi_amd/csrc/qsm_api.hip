@@ -152,7 +152,7 @@ struct Timer {
 constexpr int kGroupCode0 = 40;
 
 int timer_key(int axis, bool inv, int ld, int st, int v) {
-  return (((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st) * 2 + v;
+  return (((axis * 2 + (inv ? 1 : 0)) * 64 + ld) * 16 + st) * kVariants + v;
 }
 
 }  // namespace
@@ -165,7 +165,12 @@ struct qsm_plan_s {
   int kind[3];
   int fp32;
   const void* tw[3];
+  int variant[3] = {0, 0, 0};  // tile / radix variant of each axis's pass
 };
+
+namespace {
+int plan_measure(qsm_plan_s* p);
+}
 
 extern "C" const char* qsm_version(void) { return "qsm 0.1 (gfx950)"; }
 extern "C" const char* qsm_last_error(void) { return g_err.c_str(); }
@@ -178,7 +183,7 @@ extern "C" int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, in
   if (!is_kind(kx) || !is_kind(ky) || !is_kind(kz))
     return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: unknown r2r kind");
   if (prec != QSM_FP64 && prec != QSM_MIXED) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: bad precision");
-  if (flags != 0) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: flags must be 0");
+  if ((flags & ~QSM_PLAN_MEASURE) != 0) return fail(QSM_ERR_ARG, "qsm_r2r3d_plan: unknown flags");
   // kernels address a field as a base pointer plus a 32-bit byte offset
   if ((long)nx * ny * nz * (prec == QSM_FP64 ? 8 : 4) > (1L << 32))
     return fail(QSM_ERR_UNSUPPORTED, "qsm_r2r3d_plan: field larger than 4 GiB");
@@ -195,7 +200,21 @@ extern "C" int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, in
     p->tw[a] = twiddles(n[a], p->fp32);
     if (!p->tw[a]) { delete p; return fail(QSM_ERR_NOMEM, "qsm_r2r3d_plan: twiddle table"); }
   }
+  if (flags & QSM_PLAN_MEASURE) {
+    const int rc = plan_measure(p);
+    if (rc) { delete p; return rc; }
+  }
   *out = p;
+  return QSM_OK;
+}
+
+extern "C" int qsm_r2r3d_set_variant(qsm_plan p, int axis, int variant) {
+  if (!p || axis < 0 || axis > 2 || variant < 0 || variant >= kVariants)
+    return fail(QSM_ERR_ARG, "qsm_r2r3d_set_variant: bad argument");
+  if (!pass_variant_ok(p->fp32, p->n[axis], variant))
+    return fail(QSM_ERR_UNSUPPORTED, "qsm_r2r3d_set_variant: length %d has no variant %d",
+                p->n[axis], variant);
+  p->variant[axis] = variant;
   return QSM_OK;
 }
 
@@ -217,7 +236,7 @@ extern "C" int qsm_r2r3d_execute(qsm_plan p, const void* d_in, void* d_out, void
     A.tw = p->tw[ax];
     A.dst = kind_dst(p->kind[ax]);
     hipError_t e = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), LD_PLAIN, ST_PLAIN, A,
-                               nullptr, s);
+                               nullptr, s, nullptr, p->variant[ax]);
     if (e != hipSuccess) return fail(QSM_ERR_HIP, "pass launch failed: %s", hipGetErrorString(e));
   }
   return QSM_OK;
@@ -1271,7 +1290,8 @@ extern "C" int qsm_timing_enable(qsm_handle h, int enable) {
 
 extern "C" int qsm_timing_select(qsm_handle h, int axis, int inv, int ld, int st, int variant) {
   if (!h) return fail(QSM_ERR_ARG, "null handle");
-  if (axis < 0 || axis > 2 || ld < 0 || ld >= 64 || st < 0 || st >= 16 || variant < 0 || variant > 1)
+  if (axis < 0 || axis > 2 || ld < 0 || ld >= 64 || st < 0 || st >= 16 || variant < 0 ||
+      variant >= kVariants)
     return fail(QSM_ERR_ARG, "qsm_timing_select: bad kernel (axis %d ld %d st %d variant %d)",
                 axis, ld, st, variant);
   h->timer.reset();
@@ -1308,11 +1328,11 @@ extern "C" int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, in
   if (rank >= (int)order.size()) return fail(QSM_ERR_ARG, "qsm_timing_kernel: only %zu kernels timed", order.size());
   const int key = order[rank].second;
   const auto& a = h->timer.acc[key];
-  if (axis) *axis = key / (64 * 16 * 2) / 2;
-  if (inv) *inv = (key / (64 * 16 * 2)) % 2;
-  if (ld) *ld = (key / (16 * 2)) % 64;
-  if (st) *st = (key / 2) % 16;
-  if (variant) *variant = key % 2;
+  if (axis) *axis = key / (64 * 16 * kVariants) / 2;
+  if (inv) *inv = (key / (64 * 16 * kVariants)) % 2;
+  if (ld) *ld = (key / (16 * kVariants)) % 64;
+  if (st) *st = (key / kVariants) % 16;
+  if (variant) *variant = key % kVariants;
   if (total_ms) *total_ms = a.ms;
   if (launches) *launches = a.count;
   if (alg_bytes) *alg_bytes = a.bytes;
@@ -1346,6 +1366,9 @@ struct EnsPlan {
   hipGraphExec_t exec = nullptr;
 };
 
+// Largest number of steps one ensemble graph launch may chain.
+constexpr int kMaxGraphSteps = 64;
+
 // Side streams of an ensemble: the launches of one pass for different pencil
 // lengths touch disjoint instances, so up to kEnsSide + 1 of them run
 // side by side.  Default priority, like every stream here.
@@ -1360,6 +1383,12 @@ struct qsm_ens_s {
   hipStream_t side[kEnsSide] = {};
   bool use_graphs = true;
   std::map<std::vector<long long>, EnsPlan*> plans;
+  // multi-step graphs: graph_steps steps per launch once the step sequence
+  // repeats (keyed by the first step's signature and the step count); the
+  // signatures of the last two steps run one at a time
+  int graph_steps = 1;
+  std::vector<long long> last_key[2];
+  std::map<std::vector<long long>, hipGraphExec_t> blocks;
   std::map<int, int> variant[3][2];  // tile-width variant of plain passes [axis][inv][length]
 };
 
@@ -1487,6 +1516,8 @@ void ens_free_plan(EnsPlan* pl) {
 
 int ens_one_step(qsm_ens_s* e) {
   const auto key = step_key(e->inst[0]);
+  e->last_key[0] = e->last_key[1];
+  e->last_key[1] = key;
   auto it = e->plans.find(key);
   if (it != e->plans.end()) {
     for (auto* h : e->inst) {
@@ -1514,6 +1545,81 @@ int ens_one_step(qsm_ens_s* e) {
   hipGraphDestroy(g);
   if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble graph instantiate: %s", hipGetErrorString(err));
   HIPCHK(hipGraphLaunch(pl->exec, e->stream));
+  return QSM_OK;
+}
+
+// Signature bits of step n that depend on n alone (the rest of step_key is
+// buffer pointers, which alternate with period 2 from step 2 on).
+int phase_bits(const qsm_params& p, long long n) {
+  const bool quasi = p.variant == QSM_QUASI;
+  return (n == 1 ? 1 : 0) | (n >= 2 ? 2 : 0) | (quasi && n > p.divv_switch ? 4 : 0) |
+         (quasi && n > p.divv_switch + 1 ? 8 : 0);
+}
+
+void ens_free_blocks(qsm_ens_s* e) {
+  for (auto& kv : e->blocks) (void)hipGraphExecDestroy(kv.second);
+  e->blocks.clear();
+}
+
+// K steps (K even) as one graph launch, when the next K steps repeat the
+// last two: the signature of the coming step equals that of two steps ago
+// and no phase bit changes from two steps ago to the end of the block.  Then
+// the block's steps alternate between the two captured single-step plans,
+// and the block graph chains their launches K/2 times.  *done: the steps
+// advanced (0: not applicable, the caller steps one at a time).
+int ens_block(qsm_ens_s* e, int K, int* done) {
+  *done = 0;
+  qsm_handle_s* h0 = e->inst[0];
+  const long long n0 = h0->step + 1;
+  if (K < 2 || (K & 1) || n0 < 3 || e->last_key[0].empty()) return QSM_OK;
+  const auto key = step_key(h0);
+  if (key != e->last_key[0]) return QSM_OK;
+  const int ph = phase_bits(h0->p, n0 - 2);
+  for (long long n = n0 - 1; n < n0 + K; ++n)
+    if (phase_bits(h0->p, n) != ph) return QSM_OK;
+  auto pa = e->plans.find(e->last_key[0]), pb = e->plans.find(e->last_key[1]);
+  if (pa == e->plans.end() || pb == e->plans.end()) return QSM_OK;
+  std::vector<long long> bkey = key;
+  bkey.push_back(K);
+  auto dry_step = [&]() -> int {
+    for (auto* h : e->inst) {
+      h->dry = true;
+      const int rc = one_step(h);
+      h->dry = false;
+      if (rc) return rc;
+    }
+    return QSM_OK;
+  };
+  auto it = e->blocks.find(bkey);
+  if (it == e->blocks.end()) {
+    // capture: the bookkeeping of every step (no HIP calls) and the launches
+    // of its plan
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = QSM_OK;
+    for (int j = 0; j < K && rc == QSM_OK; ++j) {
+      rc = dry_step();
+      if (rc == QSM_OK) rc = ens_issue(e, (j & 1) ? pb->second : pa->second);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t err = hipStreamEndCapture(e->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble block capture: %s", hipGetErrorString(err));
+    hipGraphExec_t ex = nullptr;
+    err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (err != hipSuccess) return fail(QSM_ERR_HIP, "ensemble block instantiate: %s", hipGetErrorString(err));
+    it = e->blocks.emplace(bkey, ex).first;
+  } else {
+    for (int j = 0; j < K; ++j) {
+      const int rc = dry_step();
+      if (rc) return rc;
+    }
+  }
+  HIPCHK(hipGraphLaunch(it->second, e->stream));
+  // the signatures of the block's last two steps, for the next block
+  e->last_key[0] = (K & 1) ? pb->first : pa->first;
+  e->last_key[1] = (K & 1) ? pa->first : pb->first;
+  *done = K;
   return QSM_OK;
 }
 
@@ -1553,9 +1659,16 @@ extern "C" int qsm_ens_step(qsm_ens e, int nsteps) {
   if (!e || nsteps < 0) return fail(QSM_ERR_ARG, "qsm_ens_step: bad argument");
   for (auto* h : e->inst)
     if (!h->ready) return fail(QSM_ERR_STATE, "qsm_ens_step: set psi of every instance first");
-  for (int s = 0; s < nsteps; ++s) {
-    int rc = ens_one_step(e);
+  for (int s = 0; s < nsteps;) {
+    if (e->use_graphs && e->graph_steps > 1 && nsteps - s >= e->graph_steps) {
+      int done = 0;
+      const int rc = ens_block(e, e->graph_steps, &done);
+      if (rc) return rc;
+      if (done) { s += done; continue; }
+    }
+    const int rc = ens_one_step(e);
     if (rc) return rc;
+    ++s;
   }
   HIPCHK(hipStreamSynchronize(e->stream));
   if (nsteps > 0 && e->inst[0]->p.variant == QSM_STB)
@@ -1590,9 +1703,18 @@ extern "C" int qsm_ens_set_graphs(qsm_ens e, int enable) {
   return QSM_OK;
 }
 
+extern "C" int qsm_ens_set_graph_steps(qsm_ens e, int steps) {
+  if (!e || steps < 1 || (steps > 1 && (steps & 1)) || steps > kMaxGraphSteps)
+    return fail(QSM_ERR_ARG, "qsm_ens_set_graph_steps: steps must be 1 or even, at most %d",
+                kMaxGraphSteps);
+  e->graph_steps = steps;
+  return QSM_OK;
+}
+
 extern "C" int qsm_ens_destroy(qsm_ens e) {
   if (!e) return QSM_OK;
   if (e->stream) hipStreamSynchronize(e->stream);
+  ens_free_blocks(e);
   for (auto& kv : e->plans) ens_free_plan(kv.second);
   for (auto* h : e->inst) qsm_destroy(h);
   for (auto side : e->side)
@@ -1742,10 +1864,16 @@ std::string gpu_name() {
 }
 
 // Each variant is timed in kTuneRounds interleaved rounds and keeps its
-// fastest; the default width is replaced only by a variant at least 3%
-// faster, so run-to-run noise does not flip the choice.
+// fastest; the default (variant 0) is replaced only by a variant at least 3%
+// faster, the fastest of those, so run-to-run noise does not flip the
+// choice.  Variants a length does not have are timed as +inf.
 constexpr int kTuneRounds = 3;
-int tune_pick(const double ms[2]) { return ms[1] < 0.97 * ms[0] ? 1 : 0; }
+int tune_pick(const double* ms, int nv) {
+  int best = 0;
+  for (int v = 1; v < nv; ++v)
+    if (ms[v] < 0.97 * ms[0] && ms[v] < ms[best]) best = v;
+  return best;
+}
 
 // average ms of `reps` launches of fn on stream s (after one warm-up launch)
 template <class F>
@@ -1767,6 +1895,48 @@ double time_launches(hipStream_t s, int reps, F fn) {
 
 }  // namespace
 
+namespace {
+
+// QSM_PLAN_MEASURE: each axis's pass is timed in place on a scratch field
+// in every variant its length has, in interleaved rounds, and the plan keeps
+// the pick of tune_pick (the counterpart of FFTW_MEASURE at quasi.cpp:449-477,
+// which also times candidate algorithms on scratch data when planning).
+int plan_measure(qsm_plan_s* p) {
+  const size_t bytes = (size_t)p->n[0] * p->n[1] * p->n[2] * (p->fp32 ? 4 : 8);
+  void* d = nullptr;
+  HIPCHK(hipMalloc(&d, bytes));
+  hipStream_t s = nullptr;
+  int rc = QSM_OK;
+  if (hipMemsetAsync(d, 0, bytes, s) != hipSuccess) rc = fail(QSM_ERR_HIP, "plan_measure: clear");
+  for (int ax = 0; ax < 3 && rc == QSM_OK; ++ax) {
+    PassArgs A{};
+    A.geo = Geo{p->n[0], p->n[1], p->n[2], 0};
+    A.ld.code = LD_PLAIN; A.ld.a[0] = d; A.st.code = ST_PLAIN; A.st.d[0] = d;
+    A.tw = p->tw[ax];
+    A.dst = kind_dst(p->kind[ax]);
+    double ms[kVariants];
+    for (double& m : ms) m = 1e30;
+    for (int round = 0; round < kTuneRounds && rc == QSM_OK; ++round)
+      for (int v = 0; v < kVariants; ++v) {
+        if (!pass_variant_ok(p->fp32, p->n[ax], v)) continue;
+        hipError_t err = hipSuccess;
+        const double t = time_launches(s, 3, [&] {
+          hipError_t e2 = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), LD_PLAIN,
+                                      ST_PLAIN, A, nullptr, s, nullptr, v);
+          if (e2 != hipSuccess) err = e2;
+        });
+        if (err != hipSuccess) { rc = fail(QSM_ERR_HIP, "plan_measure: %s", hipGetErrorString(err)); break; }
+        ms[v] = std::min(ms[v], t);
+      }
+    if (rc == QSM_OK) p->variant[ax] = tune_pick(ms, kVariants);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && rc == QSM_OK) rc = fail(QSM_ERR_HIP, "plan_measure: sync");
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // namespace
+
 extern "C" int qsm_autotune(qsm_handle h, const char* cache_path) {
   if (!h) return fail(QSM_ERR_ARG, "qsm_autotune: null handle");
   auto cache = cache_read(cache_path);
@@ -1782,9 +1952,11 @@ extern "C" int qsm_autotune(qsm_handle h, const char* cache_path) {
       PassArgs A{};
       A.geo = h->geo; A.waves = h->waves; A.phys = h->phys; A.tw = h->tw[axis];
       A.ld.code = LD_PLAIN; A.ld.a[0] = scratch; A.st.code = ST_PLAIN; A.st.d[0] = scratch;
-      double ms[2] = {1e30, 1e30};
+      double ms[kVariants];
+      for (double& m : ms) m = 1e30;
       for (int round = 0; round < kTuneRounds; ++round)
-        for (int v = 0; v < 2; ++v) {
+        for (int v = 0; v < kVariants; ++v) {
+          if (!pass_variant_ok(h->fp32, nn[axis], v)) continue;
           hipError_t err = hipSuccess;
           const double t = time_launches(h->stream, 3, [&] {
             hipError_t e2 = launch_pass(h->fp32, nn[axis], axis, inv != 0, LD_PLAIN, ST_PLAIN, A,
@@ -1794,7 +1966,7 @@ extern "C" int qsm_autotune(qsm_handle h, const char* cache_path) {
           if (err != hipSuccess) return fail(QSM_ERR_HIP, "qsm_autotune: %s", hipGetErrorString(err));
           ms[v] = std::min(ms[v], t);
         }
-      const int bv = tune_pick(ms);
+      const int bv = tune_pick(ms, kVariants);
       h->variant[axis][inv] = bv;
       cache[key] = bv;
     }
@@ -1820,9 +1992,11 @@ extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
         const std::string key = base + "|" + std::to_string(g.first) + "|" + std::to_string(axis) + "|" + (inv ? "inv" : "fwd");
         auto it = cache.find(key);
         if (it != cache.end()) { e->variant[axis][inv][g.first] = it->second; continue; }
-        double mv[2] = {1e30, 1e30};
-        for (int rv = 0; rv < 2 * kTuneRounds; ++rv) {
-          const int v = rv % 2;  // rounds interleave the variants
+        double mv[kVariants];
+        for (double& m : mv) m = 1e30;
+        for (int rv = 0; rv < kVariants * kTuneRounds; ++rv) {
+          const int v = rv % kVariants;  // rounds interleave the variants
+          if (!pass_variant_ok(e->fp32, g.first, v)) continue;
           std::vector<PassArgs> descs;
           std::vector<int> prefix(1, 0);
           for (int i : g.second) {
@@ -1853,10 +2027,11 @@ extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
           if (err != hipSuccess) return fail(QSM_ERR_HIP, "qsm_ens_autotune: %s", hipGetErrorString(err));
           mv[v] = std::min(mv[v], ms);
         }
-        const int bv = tune_pick(mv);
+        const int bv = tune_pick(mv, kVariants);
         e->variant[axis][inv][g.first] = bv;
         cache[key] = bv;
       }
+  ens_free_blocks(e);
   for (auto& kv : e->plans) ens_free_plan(kv.second);
   e->plans.clear();
   cache_write(cache_path, cache);
@@ -1864,8 +2039,11 @@ extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
 }
 
 extern "C" int qsm_set_variant(qsm_handle h, int axis, int inv, int variant) {
-  if (!h || axis < 0 || axis > 2 || (inv != 0 && inv != 1) || (variant != 0 && variant != 1))
+  if (!h || axis < 0 || axis > 2 || (inv != 0 && inv != 1) || variant < 0 || variant >= kVariants)
     return fail(QSM_ERR_ARG, "qsm_set_variant: bad argument");
+  const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
+  if (!pass_variant_ok(h->fp32, n, variant))
+    return fail(QSM_ERR_UNSUPPORTED, "qsm_set_variant: length %d has no variant %d", n, variant);
   h->variant[axis][inv] = variant;
   for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
   h->graphs.clear();

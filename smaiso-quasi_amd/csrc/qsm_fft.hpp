@@ -306,4 +306,59 @@ struct PlanFor<double, 256> {
   static constexpr int R1 = 8;
 };
 
+// Alternative radix plans A = 0, 1 per half-length M, for the autotuner to
+// time against the default (tile variants 2 and 3 of the plain passes).
+// Each trades stages against elements per thread: fewer, larger radices
+// (more registers, fewer LDS round trips, fewer threads) or more, smaller
+// ones.  Constraints: the radices multiply to M, E is a multiple of each,
+// and M/E > 1 (at least two threads per pencil, so a strided tile of TK
+// pencils fills a wavefront).  `ok` is false where M has no alternative.
+template <typename T, int M, int A>
+struct AltPlan {
+  static constexpr bool ok = false;
+};
+#define QSM_ALT(T_, M_, A_, E_, ...)                      \
+  template <>                                             \
+  struct AltPlan<T_, M_, A_> {                            \
+    static constexpr bool ok = true;                      \
+    static constexpr int E = E_;                          \
+    using R = Radices<__VA_ARGS__>;                       \
+    static constexpr int R1 = First<__VA_ARGS__>::value;  \
+  };
+#define QSM_ALT2(M_, A_, E_, ...) QSM_ALT(float, M_, A_, E_, __VA_ARGS__) QSM_ALT(double, M_, A_, E_, __VA_ARGS__)
+QSM_ALT2(16, 0, 4, 4, 4)
+QSM_ALT2(16, 1, 8, 2, 8)
+QSM_ALT2(24, 0, 6, 3, 2, 2, 2)
+QSM_ALT2(24, 1, 12, 2, 3, 4)
+QSM_ALT2(32, 0, 16, 16, 2)
+QSM_ALT2(32, 1, 4, 4, 4, 2)
+QSM_ALT2(48, 0, 6, 3, 2, 2, 2, 2)
+QSM_ALT2(48, 1, 24, 4, 3, 4)
+QSM_ALT2(64, 0, 16, 16, 4)
+QSM_ALT2(64, 1, 4, 4, 4, 4)
+QSM_ALT2(96, 0, 6, 3, 2, 2, 2, 2, 2)
+QSM_ALT2(96, 1, 24, 4, 4, 3, 2)
+QSM_ALT2(128, 0, 8, 8, 8, 2)
+QSM_ALT2(128, 1, 4, 4, 4, 4, 2)
+QSM_ALT2(192, 0, 24, 4, 4, 4, 3)
+QSM_ALT2(192, 1, 12, 3, 4, 4, 4)
+// M = 256: the other precision's default, and a reordered E = 16 plan
+QSM_ALT(float, 256, 0, 8, 8, 8, 4)
+QSM_ALT(double, 256, 0, 16, 16, 16)
+QSM_ALT2(256, 1, 16, 4, 4, 16)
+#undef QSM_ALT2
+#undef QSM_ALT
+
+// Plan of tile variant V: 0 and 1 (full and half width) the default plan,
+// 2 and 3 the alternatives 0 and 1 where they exist (else the default).
+template <typename T, int M, int V, bool ALT = (V >= 2 && AltPlan<T, M, (V >= 2 ? V - 2 : 0)>::ok)>
+struct PlanV : PlanFor<T, M> {};
+template <typename T, int M, int V>
+struct PlanV<T, M, V, true> : AltPlan<T, M, V - 2> {};
+
+template <typename T, int M, int V>
+constexpr bool plan_distinct() {
+  return V >= 2 && AltPlan<T, M, (V >= 2 ? V - 2 : 0)>::ok;
+}
+
 }  // namespace qsm

@@ -124,8 +124,28 @@ static int tile_k_t(int n, int axis) {
 }
 
 int pass_tile_k(int prec_fp32, int n, int axis, int variant) {
+  // variants 2 and 3 change the radix plan, not the tile width
   if (variant == 1) return prec_fp32 ? tile_k_t<float, 1>(n, axis) : tile_k_t<double, 1>(n, axis);
   return prec_fp32 ? tile_k_t<float, 0>(n, axis) : tile_k_t<double, 0>(n, axis);
+}
+
+template <typename T>
+static bool variant_ok_t(int n, int variant) {
+  if (variant == 0 || variant == 1) return pass_supported(n);
+  if (variant != 2 && variant != 3) return false;
+  switch (n) {
+#define QSM_CASE(NN) \
+  case NN:           \
+    return variant == 2 ? plan_distinct<T, NN / 2, 2>() : plan_distinct<T, NN / 2, 3>();
+    QSM_FOR_EACH_N(QSM_CASE)
+#undef QSM_CASE
+    default:
+      return false;
+  }
+}
+
+bool pass_variant_ok(int prec_fp32, int n, int variant) {
+  return prec_fp32 ? variant_ok_t<float>(n, variant) : variant_ok_t<double>(n, variant);
 }
 
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g) {

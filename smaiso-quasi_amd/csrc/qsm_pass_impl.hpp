@@ -42,7 +42,7 @@ constexpr int kFuseChunk = 4;
 template <typename T, int N, int AX, int V = 0>
 struct TileCfg {
   static constexpr int M = N / 2;
-  static constexpr int E = PlanFor<T, M>::E;
+  static constexpr int E = PlanV<T, M, V>::E;
   static constexpr int TP = M / E;  // threads per pencil
   static constexpr int tk() {
     if (AX == kAxZ) {
@@ -309,7 +309,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   static_assert((XLD < 0 && !FROMH) || (INV && AX != 2), "fused outputs: strided inverse passes");
   static_assert(XLD < 0 || load_split<LD>() == load_split<XLD>(), "fused operators load alike");
   using C = TileCfg<T, N, AX, V>;
-  using PL = PlanFor<T, N / 2>;
+  using PL = PlanV<T, N / 2, V>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
   constexpr int R1 = PL::R1, NB1 = E / R1;
   using TW = TwLayout<M>;
@@ -1321,6 +1321,20 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
     if (variant == 1)
       return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN, 1>(A, nb, s, B)
                  : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN, 1>(A, nb, s, B);
+    if (variant == 2 || variant == 3) {
+      // alternative radix plans, where the length has them
+      if constexpr (plan_distinct<T, N / 2, 2>()) {
+        if (variant == 2)
+          return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN, 2>(A, nb, s, B)
+                     : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN, 2>(A, nb, s, B);
+      }
+      if constexpr (plan_distinct<T, N / 2, 3>()) {
+        if (variant == 3)
+          return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN, 3>(A, nb, s, B)
+                     : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN, 3>(A, nb, s, B);
+      }
+      return hipErrorInvalidValue;
+    }
     return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN>(A, nb, s, B)
                : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN>(A, nb, s, B);
   }

@@ -22,6 +22,7 @@ ERRORS = {-1: "ARG", -2: "UNSUPPORTED", -3: "HIP", -4: "NONFINITE", -5: "STATE",
 
 REDFT01, REDFT10, RODFT01, RODFT10 = 4, 5, 8, 9
 FP64, MIXED = 0, 1
+PLAN_MEASURE = 1  # QSM_PLAN_MEASURE
 QUASI, STB = 0, 1
 
 FIELDS = [
@@ -91,6 +92,8 @@ SIGNATURES = {
     "qsm_ens_get_field": ([_VP, _IP, _IP, _DP], _IP),
     "qsm_ens_get_diag": ([_VP, _IP, ctypes.POINTER(Diag)], _IP),
     "qsm_ens_set_graphs": ([_VP, _IP], _IP),
+    "qsm_ens_set_graph_steps": ([_VP, _IP], _IP),
+    "qsm_r2r3d_set_variant": ([_VP, _IP, _IP], _IP),
     "qsm_ens_destroy": ([_VP], _IP),
     "qsm_autotune": ([_VP, ctypes.c_char_p], _IP),
     "qsm_ens_autotune": ([_VP, ctypes.c_char_p], _IP),
@@ -144,13 +147,17 @@ def default_params(variant=QUASI, **overrides):
 class Plan:
     """3D r2r plan: FFTW semantics (unnormalised, one kind per axis)."""
 
-    def __init__(self, shape, kinds, prec=FP64):
+    def __init__(self, shape, kinds, prec=FP64, measure=False):
         self.shape = tuple(shape)
         self.prec = prec
         self.dtype = np.float64 if prec == FP64 else np.float32
         h = ctypes.c_void_p()
-        _check(lib().qsm_r2r3d_plan(*self.shape, *kinds, prec, 0, ctypes.byref(h)))
+        flags = PLAN_MEASURE if measure else 0
+        _check(lib().qsm_r2r3d_plan(*self.shape, *kinds, prec, flags, ctypes.byref(h)))
         self.h = h
+
+    def set_variant(self, axis, variant):
+        _check(lib().qsm_r2r3d_set_variant(self.h, axis, variant))
 
     def __call__(self, x):
         x = np.ascontiguousarray(x, dtype=self.dtype)
@@ -305,6 +312,9 @@ class Ensemble:
 
     def graphs(self, enable=True):
         _check(lib().qsm_ens_set_graphs(self.h, 1 if enable else 0))
+
+    def graph_steps(self, steps):
+        _check(lib().qsm_ens_set_graph_steps(self.h, steps))
 
     def autotune(self, cache_path=None):
         _check(lib().qsm_ens_autotune(self.h, os.fsencode(cache_path) if cache_path else None))

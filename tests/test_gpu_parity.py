@@ -49,6 +49,75 @@ def test_r2r_fp32(qsm_mod, oracle_mod, n):
         assert rel_l2(got, ref) < 2e-6, (n, kinds, rel_l2(got, ref))
 
 
+# lengths whose passes have the alternative radix plans (tile variants 2, 3)
+RADIX_LENGTHS = (32, 48, 64, 96, 128, 192, 256, 384, 512)
+
+
+@pytest.mark.parametrize("n", RADIX_LENGTHS)
+@pytest.mark.parametrize("axis", (0, 1, 2))
+def test_r2r_radix_variants(qsm_mod, oracle_mod, n, axis):
+    """The alternative radix plans the autotuner times (variants 2 and 3:
+    other radix orders and elements per thread) against the oracle, every
+    kind along the axis in FP64 (1e-12) and one kind set in FP32 (2e-6)."""
+    shape = _shape_for(n, axis)
+    rng = np.random.default_rng(n * 100 + axis)
+    x = rng.standard_normal(shape)
+    for v in (2, 3):
+        for kind in KINDS:
+            kinds = [5, 4, 5]
+            kinds[axis] = kind
+            print(f"r2r radix variant {v} fp64 n={n} axis={axis} kinds={kinds}", flush=True)
+            plan = qsm_mod.Plan(shape, kinds, qsm_mod.FP64)
+            plan.set_variant(axis, v)
+            got = plan(x)
+            plan.close()
+            ref = oracle_mod.r2r_3d(x, kinds)
+            assert rel_l2(got, ref) < 1e-12, (n, axis, v, kind, rel_l2(got, ref))
+        kinds = [9, 8, 5]
+        plan = qsm_mod.Plan(shape, kinds, qsm_mod.MIXED)
+        plan.set_variant(axis, v)
+        got = plan(x.astype(np.float32))
+        plan.close()
+        ref = oracle_mod.r2r_3d(x.astype(np.float32).astype(np.float64), kinds)
+        print(f"r2r radix variant {v} fp32 n={n} axis={axis}: {rel_l2(got, ref):.2e}", flush=True)
+        assert rel_l2(got, ref) < 2e-6, (n, axis, v, rel_l2(got, ref))
+
+
+def test_r2r_variant_errors(qsm_mod):
+    """A length without alternative plans refuses variants 2 and 3; a bad
+    variant or axis is an argument error."""
+    plan = qsm_mod.Plan((24, 16, 1024), [5, 4, 5], qsm_mod.FP64)
+    for ax in range(3):
+        with pytest.raises(qsm_mod.QsmError) as ei:
+            plan.set_variant(ax, 2)
+        assert ei.value.code == -2
+        plan.set_variant(ax, 1)
+        plan.set_variant(ax, 0)
+    with pytest.raises(qsm_mod.QsmError):
+        plan.set_variant(0, 4)
+    with pytest.raises(qsm_mod.QsmError):
+        plan.set_variant(3, 0)
+    plan.close()
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_r2r_plan_measure(qsm_mod, oracle_mod, prec):
+    """QSM_PLAN_MEASURE (the reference plans with FFTW_MEASURE) times every
+    variant of each axis on scratch data and keeps one; the transform stays
+    within the precision's tolerance of the oracle."""
+    shape = (96, 64, 128)
+    x = np.random.default_rng(5).standard_normal(shape)
+    kinds = [9, 4, 5]
+    plan = qsm_mod.Plan(shape, kinds, prec, measure=True)
+    xin = x if prec == 0 else x.astype(np.float32)
+    got = plan(xin)
+    plan.close()
+    ref = oracle_mod.r2r_3d(np.asarray(xin, dtype=np.float64), kinds)
+    e = rel_l2(got, ref)
+    print(f"measured plan prec {prec}: {e:.2e}", flush=True)
+    assert e < (1e-12 if prec == 0 else 2e-6)
+
+
 def test_r2r_mixed_shape_roundtrip(qsm_mod):
     shape = (48, 96, 64)
     x = np.random.default_rng(7).standard_normal(shape)
@@ -345,6 +414,40 @@ def test_quasi_ensemble_matches_single_runs(qsm_mod, prec):
 
 
 @pytest.mark.parametrize("prec", (0, 1))
+def test_ensemble_graph_steps(qsm_mod, prec):
+    """Steps chained in multi-step graph launches (4 and 16 per launch) give
+    the fields and growth rates of one graph launch per step bit for bit,
+    over step counts that start, end and straddle blocks; invalid block
+    sizes are refused."""
+    import ics
+    members = ics.ensemble_members(count=5, seed=13)
+    params = [qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_}) for d, _ in members]
+    runs = []
+    for k in (1, 4, 16):
+        ens = qsm_mod.Ensemble(params, prec)
+        ens.graph_steps(k)
+        for i, (_, psi) in enumerate(members):
+            ens.set_psi(i, psi)
+        out = []
+        for chunk in (3, 9, 2, 17, 5, 20):
+            ens.step(chunk)
+            out.append((ens.growth(), [ens.field(i, "PSI") for i in range(len(members))],
+                        ens.field(0, "VX")))
+        runs.append(out)
+        if k == 4:
+            for bad in (0, 3, 66):
+                with pytest.raises(qsm_mod.QsmError):
+                    ens.graph_steps(bad)
+        ens.close()
+    for other in runs[1:]:
+        for (g0, p0, v0), (g1, p1, v1) in zip(runs[0], other):
+            assert np.array_equal(g0, g1)
+            assert np.array_equal(v0, v1)
+            for a, b in zip(p0, p1):
+                assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", (0, 1))
 def test_tile_variants_agree(qsm_mod, prec):
     """The half-width tile variant of the plain passes (chosen by the
     autotuner) regroups pencils into workgroups without changing the
@@ -372,9 +475,41 @@ def test_tile_variants_agree(qsm_mod, prec):
         s_.step(4)
     for f in ("PSI", "PSIHAT", "RHO", "VX", "VY", "VZ", "NHAT"):
         a = base.field(f)
-        for other in (half, tuned):
-            b = other.field(f)
-            if prec == 1:
-                assert np.array_equal(a, b), f
-            else:
-                assert rel_l2(b, a) < (1e-13 if f[0] != "V" else 1e-8), (f, rel_l2(b, a))
+        b = half.field(f)
+        if prec == 1:
+            assert np.array_equal(a, b), f
+        else:
+            assert rel_l2(b, a) < (1e-13 if f[0] != "V" else 1e-8), (f, rel_l2(b, a))
+        # the autotuner may also pick an alternative radix plan, which
+        # rounds differently: the precision's own agreement.  In mixed
+        # precision past the divergence switch the transverse velocity
+        # components are round-off-sized remainders (v amplifies round-off
+        # ~1e6 there), so v is held as one vector field
+        if prec == 0 or f[0] != "V":
+            b = tuned.field(f)
+            tol = (1e-13 if f[0] != "V" else 1e-8) if prec == 0 else 1e-5
+            assert rel_l2(b, a) < tol, (f, rel_l2(b, a))
+    if prec == 1:
+        va = np.stack([base.field(f) for f in ("VX", "VY", "VZ")])
+        vb = np.stack([tuned.field(f) for f in ("VX", "VY", "VZ")])
+        print(f"tuned mixed v as a vector: {rel_l2(vb, va):.2e}", flush=True)
+        assert rel_l2(vb, va) < 1e-2
+
+
+def test_step_radix_variants(qsm_mod, oracle_mod):
+    """The FP64 step with every plain pass forced to an alternative radix
+    plan (variant 2 on x, 3 on y, 2 on z, both directions) against the
+    oracle, every intermediate field after three steps with D and the
+    pressure term live: 1e-10, and 1e-7 for the fields that amplify
+    round-off by ~1e6 once D is live."""
+    sim, orc = _pair(qsm_mod, oracle_mod, shape=(64, 48, 96), prec=0, compat_ranks=4,
+                     divv_switch=1)
+    for ax, v in ((0, 2), (1, 3), (2, 2)):
+        for inv in range(2):
+            sim.set_variant(ax, inv, v)
+    tols = {f: 1e-7 for f in ("VX", "VY", "VZ", "P", "DIVV", "DHAT")}
+    for s in range(3):
+        sim.step(1)
+        orc.step(1)
+        _cmp_fields(sim, orc, STEP_FIELDS, 1e-10, f"radix variants step{s + 1}", tols)
+    sim.close()
