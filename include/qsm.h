@@ -205,6 +205,17 @@ int qsm_ens_set_graphs(qsm_ens e, int enable);
  * divv_switch), `steps` steps are captured once as one graph and replayed
  * per launch; results are bit for bit those of one graph per step. */
 int qsm_ens_set_graph_steps(qsm_ens e, int steps);
+/* device time of an ensemble's batched launches (no counterpart in the
+ * reference): while enabled, steps launch eagerly with HIP events around
+ * every batched kernel on the stream it runs on (launches of one pass for
+ * different pencil lengths overlap on side streams, so their times add up
+ * to more than the wall time).  Enabling resets the counters.  The rank-th
+ * kernel (0 = most device time): the pass kernel as qsm_timing_kernel
+ * reports it (load code 63: the batched FP64 finish of the mean-mode sums),
+ * its pencil length n, total time, launches and algorithmic bytes. */
+int qsm_ens_timing_enable(qsm_ens e, int enable);
+int qsm_ens_timing_kernel(qsm_ens e, int rank, int* axis, int* inv, int* ld, int* st, int* variant,
+                          int* n, double* total_ms, long long* launches, double* alg_bytes);
 int qsm_ens_destroy(qsm_ens e);
 
 /* ---- autotuner: times the variants of the plain transform passes on the

@@ -337,6 +337,7 @@ enum Dsc {
 struct PassRec {
   int kind;  // 0: pass, 1: FP64 finish of per-block partial sums
   int axis, n, inv, ld, st, variant;
+  double bytes;  // algorithmic HBM bytes of the pass (op_bytes)
   PassArgs A;
   const double* part;
   long nparts;
@@ -445,6 +446,7 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
     PassRec r{};
     r.kind = 0; r.axis = axis; r.n = n; r.inv = inv; r.ld = L.code; r.st = S.code; r.A = A;
     r.variant = var;
+    r.bytes = op_bytes(h, L, S, first_pass);
     h->rec->push_back(r);
     return QSM_OK;
   }
@@ -1352,6 +1354,7 @@ struct EnsLaunch {
   int kind = 0, axis = 0, n = 0, inv = 0, ld = 0, st = 0, count = 0, variant = 0;
   int q = 0;                  // launch of the instances' step sequence this serves
   long total = 0;
+  double bytes = 0;           // algorithmic HBM bytes over the launch's instances
   PassArgs* descs = nullptr;  // device
   int* prefix = nullptr;      // device
   const double** parts = nullptr;  // device (finish)
@@ -1389,6 +1392,9 @@ struct qsm_ens_s {
   int graph_steps = 1;
   std::vector<long long> last_key[2];
   std::map<std::vector<long long>, hipGraphExec_t> blocks;
+  // per-launch HIP-event timing (eager launches while on), keyed by
+  // ens_timer_key: the pass kernel and its pencil length
+  Timer timer;
   std::map<int, int> variant[3][2];  // tile-width variant of plain passes [axis][inv][length]
 };
 
@@ -1404,8 +1410,31 @@ int upload(EnsPlan* pl, const std::vector<X>& v, X** out) {
   return QSM_OK;
 }
 
-// one batched launch of a plan on stream s
-hipError_t ens_launch(const qsm_ens_s* e, const EnsLaunch& L, hipStream_t s) {
+// timer key of an ensemble launch: the pass kernel (timer_key, with load
+// code 63 for the batched FP64 finish) and its pencil length
+constexpr int kEnsLenMod = 2048;
+int ens_timer_key(const EnsLaunch& L) {
+  const int base = L.kind == 0 ? timer_key(L.axis, L.inv != 0, L.ld, L.st, L.variant)
+                               : timer_key(0, false, 63, 0, 0);
+  return base * kEnsLenMod + (L.kind == 0 ? L.n : 0);
+}
+
+hipError_t ens_launch_raw(const qsm_ens_s* e, const EnsLaunch& L, hipStream_t s);
+
+// one batched launch of a plan on stream s, with events around it while the
+// ensemble's timer is on
+hipError_t ens_launch(qsm_ens_s* e, const EnsLaunch& L, hipStream_t s) {
+  Timer& T = e->timer;
+  if (!T.on) return ens_launch_raw(e, L, s);
+  hipEvent_t ea = T.get(), eb = T.get();
+  (void)hipEventRecord(ea, s);
+  const hipError_t err = ens_launch_raw(e, L, s);
+  (void)hipEventRecord(eb, s);
+  T.recs.push_back({ea, eb, ens_timer_key(L), L.bytes});
+  return err;
+}
+
+hipError_t ens_launch_raw(const qsm_ens_s* e, const EnsLaunch& L, hipStream_t s) {
   if (L.kind == 0) {
     BatchLaunch B{L.descs, L.prefix, L.count, L.total};
     PassArgs dummy{};
@@ -1493,6 +1522,7 @@ int ens_build(qsm_ens_s* e, EnsPlan* pl) {
       L.q = (int)q;
       L.variant = var;
       L.count = (int)g.second.size(); L.total = prefix.back();
+      for (int i : g.second) L.bytes += recs[i][q].bytes;
       int rc = upload(pl, descs, &L.descs);
       if (!rc) rc = upload(pl, prefix, &L.prefix);
       if (rc) return rc;
@@ -1527,14 +1557,14 @@ int ens_one_step(qsm_ens_s* e) {
       if (rc) return rc;
     }
     EnsPlan* pl = it->second;
-    if (pl->exec) { HIPCHK(hipGraphLaunch(pl->exec, e->stream)); return QSM_OK; }
+    if (pl->exec && !e->timer.on) { HIPCHK(hipGraphLaunch(pl->exec, e->stream)); return QSM_OK; }
     return ens_issue(e, pl);
   }
   auto* pl = new EnsPlan;
   int rc = ens_build(e, pl);  // also advances every instance's bookkeeping
   if (rc) { ens_free_plan(pl); return rc; }
   e->plans[key] = pl;
-  if (!e->use_graphs) return ens_issue(e, pl);
+  if (!e->use_graphs || e->timer.on) return ens_issue(e, pl);
   HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   rc = ens_issue(e, pl);
   hipGraph_t g = nullptr;
@@ -1660,7 +1690,7 @@ extern "C" int qsm_ens_step(qsm_ens e, int nsteps) {
   for (auto* h : e->inst)
     if (!h->ready) return fail(QSM_ERR_STATE, "qsm_ens_step: set psi of every instance first");
   for (int s = 0; s < nsteps;) {
-    if (e->use_graphs && e->graph_steps > 1 && nsteps - s >= e->graph_steps) {
+    if (e->use_graphs && !e->timer.on && e->graph_steps > 1 && nsteps - s >= e->graph_steps) {
       int done = 0;
       const int rc = ens_block(e, e->graph_steps, &done);
       if (rc) return rc;
@@ -1700,6 +1730,37 @@ extern "C" int qsm_ens_get_diag(qsm_ens e, int index, qsm_diag* d) {
 extern "C" int qsm_ens_set_graphs(qsm_ens e, int enable) {
   if (!e) return fail(QSM_ERR_ARG, "qsm_ens_set_graphs: null ensemble");
   e->use_graphs = enable != 0;
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_timing_enable(qsm_ens e, int enable) {
+  if (!e) return fail(QSM_ERR_ARG, "qsm_ens_timing_enable: null ensemble");
+  e->timer.reset();
+  e->timer.on = enable != 0;
+  return QSM_OK;
+}
+
+extern "C" int qsm_ens_timing_kernel(qsm_ens e, int rank, int* axis, int* inv, int* ld, int* st,
+                                     int* variant, int* n, double* total_ms, long long* launches,
+                                     double* alg_bytes) {
+  if (!e || rank < 0) return fail(QSM_ERR_ARG, "qsm_ens_timing_kernel: bad argument");
+  e->timer.drain();
+  std::vector<std::pair<double, int>> order;
+  for (const auto& kv : e->timer.acc) order.push_back({-kv.second.ms, kv.first});
+  std::sort(order.begin(), order.end());
+  if (rank >= (int)order.size())
+    return fail(QSM_ERR_ARG, "qsm_ens_timing_kernel: only %zu kernels timed", order.size());
+  const int full = order[rank].second, key = full / kEnsLenMod;
+  const auto& a = e->timer.acc[full];
+  if (axis) *axis = key / (64 * 16 * kVariants) / 2;
+  if (inv) *inv = (key / (64 * 16 * kVariants)) % 2;
+  if (ld) *ld = (key / (16 * kVariants)) % 64;
+  if (st) *st = (key / kVariants) % 16;
+  if (variant) *variant = key % kVariants;
+  if (n) *n = full % kEnsLenMod;
+  if (total_ms) *total_ms = a.ms;
+  if (launches) *launches = a.count;
+  if (alg_bytes) *alg_bytes = a.bytes;
   return QSM_OK;
 }
 

@@ -25,6 +25,14 @@ FP64, MIXED = 0, 1
 PLAN_MEASURE = 1  # QSM_PLAN_MEASURE
 QUASI, STB = 0, 1
 
+# names of the fused load / store operator codes (qsm_ops.hpp LoadCode,
+# StoreCode) as the timing entry points report them
+LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "lapgrad_x",
+            "lapgrad_y", "lapgrad_z", "sh", "rhostar", "divv",
+            "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
+            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3", "stress3_divv"]
+STORE_OPS = ["plain", "filter", "update", "update_stb"]
+
 FIELDS = [
     "PSI", "RHO", "VX", "VY", "VZ", "P", "DIVV", "PSIHAT", "F", "G", "GX", "GY", "GZ",
     "RX", "RY", "RZ", "LX", "LY", "LZ", "RHOHAT", "NHAT", "PSIHAT_OLD", "PSI_OLD",
@@ -93,6 +101,9 @@ SIGNATURES = {
     "qsm_ens_get_diag": ([_VP, _IP, ctypes.POINTER(Diag)], _IP),
     "qsm_ens_set_graphs": ([_VP, _IP], _IP),
     "qsm_ens_set_graph_steps": ([_VP, _IP], _IP),
+    "qsm_ens_timing_enable": ([_VP, _IP], _IP),
+    "qsm_ens_timing_kernel": ([_VP, _IP] + [ctypes.POINTER(_IP)] * 6 +
+                              [_DP, ctypes.POINTER(ctypes.c_longlong), _DP], _IP),
     "qsm_r2r3d_set_variant": ([_VP, _IP, _IP], _IP),
     "qsm_ens_destroy": ([_VP], _IP),
     "qsm_autotune": ([_VP, ctypes.c_char_p], _IP),
@@ -315,6 +326,22 @@ class Ensemble:
 
     def graph_steps(self, steps):
         _check(lib().qsm_ens_set_graph_steps(self.h, steps))
+
+    def timing(self, enable=True):
+        _check(lib().qsm_ens_timing_enable(self.h, 1 if enable else 0))
+
+    def timing_kernel(self, rank=0):
+        """The rank-th timed batched kernel: its pass fields, pencil length n,
+        total ms, launches and algorithmic bytes."""
+        iv = [ctypes.c_int() for _ in range(6)]
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        cnt = ctypes.c_longlong()
+        _check(lib().qsm_ens_timing_kernel(self.h, rank, *[ctypes.byref(x) for x in iv],
+                                           ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)))
+        keys = ("axis", "inv", "ld", "st", "variant", "n")
+        out = {k: v.value for k, v in zip(keys, iv)}
+        out.update(ms=ms.value, launches=cnt.value, bytes=by.value)
+        return out
 
     def autotune(self, cache_path=None):
         _check(lib().qsm_ens_autotune(self.h, os.fsencode(cache_path) if cache_path else None))

@@ -10,8 +10,9 @@ difference of psi, rho and v (as a vector) and the relative energy and mass
 differences every --every steps; --compare-at K compares at step K only (the
 FP64 handle is then freed and the mixed run continues alone).  --divv-switch
 S overrides the workload's divv_switch (the step after which D is computed,
-quasi.cpp:1014-1965), so a short run reaches the later regime.  Output: one
-JSON object per report line on stdout.
+quasi.cpp:1014-1965), so a short run reaches the later regime.  --autotune picks the plain
+passes' tile and radix variants first, as bench.py does.  Output: one JSON
+object per report line on stdout.
 """
 import argparse
 import json
@@ -55,6 +56,8 @@ def main():
     ap.add_argument("--compare", action="store_true")
     ap.add_argument("--compare-at", type=int, default=0)
     ap.add_argument("--divv-switch", type=int, default=None)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the plain passes' tile and radix variants first (as bench.py does)")
     args = ap.parse_args()
     w = ics.WORKLOADS[args.workload]
     steps = args.steps or w["steps"]
@@ -65,6 +68,8 @@ def main():
     psi = ics.workload_psi(args.workload)
     prec = qsm.MIXED if args.prec == "mixed" else qsm.FP64
     sim = qsm.Sim(p, prec)
+    if args.autotune:
+        sim.autotune()
     sim.set_psi(psi)
     ref = None
     if args.compare or args.compare_at:

@@ -447,6 +447,42 @@ def test_ensemble_graph_steps(qsm_mod, prec):
                 assert np.array_equal(a, b)
 
 
+def test_ensemble_timing(qsm_mod):
+    """Per-launch event timing of an ensemble: every batched launch of the
+    timed steps is counted once (the launches of graph-replayed steps are not
+    timed), the device time is positive, and the timed (eager) steps leave
+    the fields those steps give when replayed from graphs."""
+    import ics
+    members = ics.ensemble_members(count=4, seed=17)
+    params = [qsm_mod.Params(**{k: d[k] for k, _ in qsm_mod.Params._fields_}) for d, _ in members]
+    runs = []
+    for timed in (False, True):
+        ens = qsm_mod.Ensemble(params, qsm_mod.MIXED)
+        for i, (_, psi) in enumerate(members):
+            ens.set_psi(i, psi)
+        ens.step(3)
+        if timed:
+            ens.timing(True)
+        ens.step(2)
+        if timed:
+            ks, rank = [], 0
+            while True:
+                try:
+                    ks.append(ens.timing_kernel(rank))
+                except qsm_mod.QsmError:
+                    break
+                rank += 1
+            assert ks and all(k["ms"] > 0 and k["launches"] % 2 == 0 for k in ks)
+            assert ks == sorted(ks, key=lambda k: -k["ms"])
+            lens = {d[a] for d, _ in members for a in ("nx", "ny", "nz")}
+            assert {k["n"] for k in ks if k["ld"] != 63} <= lens
+            ens.timing(False)
+        runs.append([ens.field(i, "PSI") for i in range(len(members))])
+        ens.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", (0, 1))
 def test_tile_variants_agree(qsm_mod, prec):
     """The half-width tile variant of the plain passes (chosen by the
