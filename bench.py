@@ -94,6 +94,9 @@ def kernel_symbol(k, shape, fp32):
         return f"void qsm::k_group<{t}, {n_ax}, {k['ld'] - 40}>(qsm::GroupArgs)"
     if k["ld"] < len(LOAD_OPS) and LOAD_OPS[k["ld"]] in ("stress3", "stress_stb3", "stress3_divv"):
         return f"void qsm::k_multi<{t}, {n_ax}, {k['ld']}>(qsm::MultiArgs)"
+    if k["variant"] >= 4:  # software-pipelined persistent plain pass
+        return (f"void qsm::k_pass_pipe<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+                f"{k['variant']}>(qsm::PassArgs, int)")
     return (f"void qsm::k_pass<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
             f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
 

@@ -224,12 +224,15 @@ int qsm_ens_destroy(qsm_ens e);
  * ensembles) the fastest variant that is at least 3% faster than the
  * default, else the default.  Variants: 0 default tile and radix plan, 1
  * half-width tiles, 2 and 3 alternative radix plans (pencil lengths 32-512;
- * they round differently, at the level of the transform's round-off).
+ * they round differently, at the level of the transform's round-off), 4 and
+ * 5 the software-pipelined forms of 0 and 2 (inverse passes whose tile fills
+ * a CU's LDS alone: FP64 x at 512, FP32 x at 1024; single handles and
+ * plans, not ensembles).
  * cache_path (may be NULL): JSON file of earlier choices, read and updated. */
 int qsm_autotune(qsm_handle h, const char* cache_path);
-/* set the variant (0-3, as above) of the plain passes along axis (0 x, 1 y,
+/* set the variant (0-5, as above) of the plain passes along axis (0 x, 1 y,
  * 2 z) in direction inv (1: DCT-III/DST-III) by hand; QSM_ERR_UNSUPPORTED
- * where that axis's length has no such variant */
+ * where that axis's length and direction have no such variant */
 int qsm_set_variant(qsm_handle h, int axis, int inv, int variant);
 int qsm_ens_autotune(qsm_ens e, const char* cache_path);
 

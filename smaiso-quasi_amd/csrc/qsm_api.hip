@@ -211,7 +211,7 @@ extern "C" int qsm_r2r3d_plan(int nx, int ny, int nz, int kx, int ky, int kz, in
 extern "C" int qsm_r2r3d_set_variant(qsm_plan p, int axis, int variant) {
   if (!p || axis < 0 || axis > 2 || variant < 0 || variant >= kVariants)
     return fail(QSM_ERR_ARG, "qsm_r2r3d_set_variant: bad argument");
-  if (!pass_variant_ok(p->fp32, p->n[axis], variant))
+  if (!pass_variant_ok(p->fp32, p->n[axis], variant, axis, kind_inv(p->kind[axis])))
     return fail(QSM_ERR_UNSUPPORTED, "qsm_r2r3d_set_variant: length %d has no variant %d",
                 p->n[axis], variant);
   p->variant[axis] = variant;
@@ -1979,7 +1979,7 @@ int plan_measure(qsm_plan_s* p) {
     for (double& m : ms) m = 1e30;
     for (int round = 0; round < kTuneRounds && rc == QSM_OK; ++round)
       for (int v = 0; v < kVariants; ++v) {
-        if (!pass_variant_ok(p->fp32, p->n[ax], v)) continue;
+        if (!pass_variant_ok(p->fp32, p->n[ax], v, ax, kind_inv(p->kind[ax]))) continue;
         hipError_t err = hipSuccess;
         const double t = time_launches(s, 3, [&] {
           hipError_t e2 = launch_pass(p->fp32, p->n[ax], ax, kind_inv(p->kind[ax]), LD_PLAIN,
@@ -2017,7 +2017,7 @@ extern "C" int qsm_autotune(qsm_handle h, const char* cache_path) {
       for (double& m : ms) m = 1e30;
       for (int round = 0; round < kTuneRounds; ++round)
         for (int v = 0; v < kVariants; ++v) {
-          if (!pass_variant_ok(h->fp32, nn[axis], v)) continue;
+          if (!pass_variant_ok(h->fp32, nn[axis], v, axis, inv != 0)) continue;
           hipError_t err = hipSuccess;
           const double t = time_launches(h->stream, 3, [&] {
             hipError_t e2 = launch_pass(h->fp32, nn[axis], axis, inv != 0, LD_PLAIN, ST_PLAIN, A,
@@ -2053,11 +2053,11 @@ extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
         const std::string key = base + "|" + std::to_string(g.first) + "|" + std::to_string(axis) + "|" + (inv ? "inv" : "fwd");
         auto it = cache.find(key);
         if (it != cache.end()) { e->variant[axis][inv][g.first] = it->second; continue; }
-        double mv[kVariants];
+        double mv[kBatchVariants];
         for (double& m : mv) m = 1e30;
-        for (int rv = 0; rv < kVariants * kTuneRounds; ++rv) {
-          const int v = rv % kVariants;  // rounds interleave the variants
-          if (!pass_variant_ok(e->fp32, g.first, v)) continue;
+        for (int rv = 0; rv < kBatchVariants * kTuneRounds; ++rv) {
+          const int v = rv % kBatchVariants;  // rounds interleave the variants
+          if (!pass_variant_ok(e->fp32, g.first, v, axis, inv != 0)) continue;
           std::vector<PassArgs> descs;
           std::vector<int> prefix(1, 0);
           for (int i : g.second) {
@@ -2088,7 +2088,7 @@ extern "C" int qsm_ens_autotune(qsm_ens e, const char* cache_path) {
           if (err != hipSuccess) return fail(QSM_ERR_HIP, "qsm_ens_autotune: %s", hipGetErrorString(err));
           mv[v] = std::min(mv[v], ms);
         }
-        const int bv = tune_pick(mv, kVariants);
+        const int bv = tune_pick(mv, kBatchVariants);
         e->variant[axis][inv][g.first] = bv;
         cache[key] = bv;
       }
@@ -2103,7 +2103,7 @@ extern "C" int qsm_set_variant(qsm_handle h, int axis, int inv, int variant) {
   if (!h || axis < 0 || axis > 2 || (inv != 0 && inv != 1) || variant < 0 || variant >= kVariants)
     return fail(QSM_ERR_ARG, "qsm_set_variant: bad argument");
   const int n = axis == 0 ? h->p.nx : (axis == 1 ? h->p.ny : h->p.nz);
-  if (!pass_variant_ok(h->fp32, n, variant))
+  if (!pass_variant_ok(h->fp32, n, variant, axis, inv != 0))
     return fail(QSM_ERR_UNSUPPORTED, "qsm_set_variant: length %d has no variant %d", n, variant);
   h->variant[axis][inv] = variant;
   for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);

@@ -349,16 +349,23 @@ QSM_ALT2(256, 1, 16, 4, 4, 16)
 #undef QSM_ALT2
 #undef QSM_ALT
 
-// Plan of tile variant V: 0 and 1 (full and half width) the default plan,
-// 2 and 3 the alternatives 0 and 1 where they exist (else the default).
-template <typename T, int M, int V, bool ALT = (V >= 2 && AltPlan<T, M, (V >= 2 ? V - 2 : 0)>::ok)>
+// Tile variants of the plain passes: 0 default, 1 half-width tiles, 2 and 3
+// the alternative radix plans, 4 and 5 the software-pipelined forms of 0
+// and 2 (same tile and plan; qsm_pass_impl.hpp, k_pass_pipe).  plan_base
+// gives the variant whose tile and plan a variant uses.
+constexpr int plan_base(int V) { return V == 4 ? 0 : (V == 5 ? 2 : V); }
+
+// Plan of tile variant V: the default plan, or for plan_base(V) = 2, 3 the
+// alternatives 0 and 1 where they exist (else the default).
+template <typename T, int M, int V,
+          bool ALT = (plan_base(V) >= 2 && AltPlan<T, M, (plan_base(V) >= 2 ? plan_base(V) - 2 : 0)>::ok)>
 struct PlanV : PlanFor<T, M> {};
 template <typename T, int M, int V>
-struct PlanV<T, M, V, true> : AltPlan<T, M, V - 2> {};
+struct PlanV<T, M, V, true> : AltPlan<T, M, plan_base(V) - 2> {};
 
 template <typename T, int M, int V>
 constexpr bool plan_distinct() {
-  return V >= 2 && AltPlan<T, M, (V >= 2 ? V - 2 : 0)>::ok;
+  return (V == 2 || V == 3) && AltPlan<T, M, (V >= 2 && V <= 3 ? V - 2 : 0)>::ok;
 }
 
 }  // namespace qsm

@@ -44,7 +44,7 @@ struct BatchLaunch {
 // B != nullptr launches the batched kernel over B's instances instead; A is
 // then ignored (every instance's arguments come from B->descs).
 // variant: tile variant of plain passes (0 default, 1 half width, 2 and 3
-// alternative radix plans); ignored for fused operators.
+// alternative radix plans, 4 and 5 pipelined); ignored for fused operators.
 hipError_t launch_pass(int prec_fp32, int n, int axis, bool inv, int ld, int st, const PassArgs& A,
                        long* nblocks, hipStream_t s, const BatchLaunch* B = nullptr,
                        int variant = 0);
@@ -52,10 +52,14 @@ long pass_tiles(int prec_fp32, int n, int axis, const Geo& g, int variant = 0);
 bool pass_supported(int n);
 bool step_supported(int n);
 int pass_tile_k(int prec_fp32, int n, int axis, int variant = 0);
-// variant of the plain passes of length n: 0 default, 1 half-width tiles,
-// 2 and 3 the alternative radix plans (only where the length has them)
-bool pass_variant_ok(int prec_fp32, int n, int variant);
-constexpr int kVariants = 4;
+// variant of the plain passes of length n along axis in direction inv: 0
+// default, 1 half-width tiles, 2 and 3 the alternative radix plans (only
+// where the length has them), 4 and 5 the software-pipelined forms of 0 and
+// 2 (only single, not batched, inverse passes whose tile fills a CU's LDS
+// alone)
+bool pass_variant_ok(int prec_fp32, int n, int variant, int axis, bool inv);
+constexpr int kVariants = 6;
+constexpr int kBatchVariants = 4;  // variants a batched (ensemble) pass has
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 
 // Group passes.  One kernel runs the first (x) pass of several inverse

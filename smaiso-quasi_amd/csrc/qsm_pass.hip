@@ -129,14 +129,26 @@ int pass_tile_k(int prec_fp32, int n, int axis, int variant) {
   return prec_fp32 ? tile_k_t<float, 0>(n, axis) : tile_k_t<double, 0>(n, axis);
 }
 
+template <typename T, int NN, int AX>
+static bool variant_ok_n(int variant, bool inv) {
+  switch (variant) {
+    case 0: case 1: return true;
+    case 2: return plan_distinct<T, NN / 2, 2>();
+    case 3: return plan_distinct<T, NN / 2, 3>();
+    case 4: return inv && pipe_ok<T, NN, AX, true, 4>();
+    case 5: return inv && pipe_ok<T, NN, AX, true, 5>();
+    default: return false;
+  }
+}
+
 template <typename T>
-static bool variant_ok_t(int n, int variant) {
-  if (variant == 0 || variant == 1) return pass_supported(n);
-  if (variant != 2 && variant != 3) return false;
+static bool variant_ok_t(int n, int variant, int axis, bool inv) {
   switch (n) {
-#define QSM_CASE(NN) \
-  case NN:           \
-    return variant == 2 ? plan_distinct<T, NN / 2, 2>() : plan_distinct<T, NN / 2, 3>();
+#define QSM_CASE(NN)                                                                  \
+  case NN:                                                                            \
+    return axis == 0 ? variant_ok_n<T, NN, 0>(variant, inv)                           \
+                     : (axis == 1 ? variant_ok_n<T, NN, 1>(variant, inv)              \
+                                  : variant_ok_n<T, NN, 2>(variant, inv));
     QSM_FOR_EACH_N(QSM_CASE)
 #undef QSM_CASE
     default:
@@ -144,8 +156,10 @@ static bool variant_ok_t(int n, int variant) {
   }
 }
 
-bool pass_variant_ok(int prec_fp32, int n, int variant) {
-  return prec_fp32 ? variant_ok_t<float>(n, variant) : variant_ok_t<double>(n, variant);
+bool pass_variant_ok(int prec_fp32, int n, int variant, int axis, bool inv) {
+  if (axis < 0 || axis > 2) return false;
+  return prec_fp32 ? variant_ok_t<float>(n, variant, axis, inv)
+                   : variant_ok_t<double>(n, variant, axis, inv);
 }
 
 long pass_blocks(int prec_fp32, int n, int axis, const Geo& g) {

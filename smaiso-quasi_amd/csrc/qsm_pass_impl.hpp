@@ -20,6 +20,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "qsm_fft.hpp"
 #include "qsm_kernels.h"
 #include "qsm_ops.hpp"
@@ -298,15 +300,23 @@ constexpr int walk_count() {
 //             from the same reads;
 //   FROMH     the tile is not loaded: held[s] (from an XLD pass of this
 //             tile) is the input at walk element s.
+//   PF        (with FROMH) called once the tile is in LDS, when held[] is
+//             free again: the pipelined passes load the next tile into it
+struct NoPrefetch {
+  __device__ void operator()() const {}
+};
+
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V, class SM, bool NTS = false,
-          int XLD = -1, bool FROMH = false>
+          int XLD = -1, bool FROMH = false, class PF = NoPrefetch>
 __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Phys& Ph,
                                           const LoadOp& Lop, const StoreOp& Sop,
                                           const void* twp, double* part, const int dstf,
                                           const int tile, SM& sm, const bool stage = true,
                                           const int tid = (int)threadIdx.x,
-                                          const LoadOp* xop = nullptr, T* held = nullptr) {
-  static_assert((XLD < 0 && !FROMH) || (INV && AX != 2), "fused outputs: strided inverse passes");
+                                          const LoadOp* xop = nullptr, T* held = nullptr,
+                                          const PF& prefetch = PF{}) {
+  static_assert(XLD < 0 || (INV && AX != 2), "fused outputs: strided inverse passes");
+  static_assert(!FROMH || AX != 2, "held inputs: strided passes");
   static_assert(XLD < 0 || load_split<LD>() == load_split<XLD>(), "fused operators load alike");
   using C = TileCfg<T, N, AX, V>;
   using PL = PlanV<T, N / 2, V>;
@@ -433,7 +443,8 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       // consecutive rows of a strided axis); the loads of a chunk are all
       // issued before any of its values is used
       constexpr int PER = TK * N / BLOCK;
-      constexpr int CH = load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER;
+      constexpr int CH = FROMH ? PER : (load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER);
+      static_assert(!FROMH || PER == walk_count<T, N, AX, V>(), "walk length");
       WalkPos<AX, N, TK, TKP, BLOCK, 1> wp;
       wp.init(tid);
 #pragma unroll 1
@@ -445,7 +456,8 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
           Ix idx;
           bool ok;
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
-          if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
+          if constexpr (FROMH) xs[c] = held[c];  // one chunk (CH = PER): s0 = 0
+          else if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
           else xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
         }
         // values of rows or columns outside the field (read from a valid
@@ -468,6 +480,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
         }
       }
       __syncthreads();
+      if constexpr (FROMH) prefetch();
       gather<T, M, E, TKP, R1>(v, buf, p, t);
     }
     run_fft<T, N, M, E, TKP, -1, true>(typename PL::R{}, v, buf, p, t, ts);
@@ -669,6 +682,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       }
     }
     __syncthreads();
+    if constexpr (FROMH) prefetch();
     // ---- DCT-III pre-processing straight into the stage-1 inputs:
     // Z_m = (V_m + V_{m+M}) + i e^{2 pi i m/N} (V_m - V_{m+M}),
     // V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m}).
@@ -985,6 +999,66 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
       A.geo, A.waves, A.phys, A.ld, A.st, A.tw, A.part, A.dst,
       tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
 }
+
+// ---- software-pipelined plain inverse passes (variants 4 and 5).  Where a tile
+// fills a CU's LDS alone (FP64 x passes at N = 512: 256-byte rows of 32
+// columns, 135 KB), one workgroup per CU runs its load, transform and store
+// phases one after another and the CU's memory traffic stops during the
+// transform.  A persistent workgroup instead loads the walk of its next tile
+// into registers before transforming the current one (from those registers
+// into LDS, pass_tile's FROMH mode), so the next tile's loads are in flight
+// while this one is transformed and stored.  Tiles are disjoint, so passes
+// may run in place.
+// Inverse passes only: a forward pass keeps its DCT-II post-processing
+// outputs in registers for the direct strided stores, and with the next
+// tile's walk held beside them it spills (FP64 x at 512: 66-94 VGPRs
+// spilled, 907-998 us per pass against 571 unpipelined).
+template <typename T, int N, int AX, bool INV, int V>
+constexpr bool pipe_ok() {
+  return INV && AX != kAxZ && (V == 4 || (V == 5 && plan_distinct<T, N / 2, 2>())) &&
+         sizeof(SmemFor<T, N, AX, INV, LD_PLAIN, ST_PLAIN, V>) > 81920;
+}
+
+// the plain load walk of one tile into held[] (pass_tile's load order)
+template <typename T, int N, int AX, int V>
+__device__ __forceinline__ void walk_prefetch(const PassArgs& A, int tile, int tid, T* held) {
+  using C = TileCfg<T, N, AX, V>;
+  TileGeo<AX, N, C::TK, C::BLOCK> tg;
+  tg.init(A.geo, tile, tid);
+#pragma unroll
+  for (int s = 0; s < walk_count<T, N, AX, V>(); ++s)
+    held[s] = ld_at<T>(A.ld.a[0], tg.wbase, s * (int)tg.wstep);
+}
+
+template <typename T, int N, int AX, bool INV, int V>
+__global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass_pipe(const PassArgs A,
+                                                                            const int ntiles) {
+  static_assert(pipe_ok<T, N, AX, INV, V>(), "pipelined pass");
+  __shared__ SmemFor<T, N, AX, INV, LD_PLAIN, ST_PLAIN, V> sm;
+  constexpr int PER = walk_count<T, N, AX, V>();
+  const int tid = (int)threadIdx.x, nb = (int)gridDim.x;
+  T held[PER];  // the walk of the tile being transformed, then of the next
+  int vb = (int)blockIdx.x;  // virtual block: the tile order of a one-shot launch
+  if (vb < ntiles) walk_prefetch<T, N, AX, V>(A, tile_order<T, N, AX, V>(A.geo, vb, ntiles), tid, held);
+  bool first = true;
+  for (; vb < ntiles; vb += nb) {
+    const int nvb = vb + nb;
+    // issued once this tile's values are in LDS; in flight during its
+    // transform and stores (barriers wait on LDS operations only)
+    auto next = [&]() {
+      if (nvb < ntiles)
+        walk_prefetch<T, N, AX, V>(A, tile_order<T, N, AX, V>(A.geo, nvb, ntiles), tid, held);
+    };
+    pass_tile<T, N, AX, INV, LD_PLAIN, ST_PLAIN, V, decltype(sm), false, -1, true, decltype(next)>(
+        A.geo, A.waves, A.phys, A.ld, A.st, A.tw, nullptr, A.dst,
+        tile_order<T, N, AX, V>(A.geo, vb, ntiles), sm, first, tid, nullptr, held, next);
+    first = false;
+    __syncthreads();  // the next tile rewrites LDS
+  }
+}
+
+template <typename T, int N, int AX, bool INV, int V>
+hipError_t launch_pipe(const PassArgs& A, long* nblocks_out, hipStream_t s);
 
 // ---- group passes (the x pass of inverse transforms): the operators of
 // each group in output order, the workgroups that share each tile in FP32
@@ -1309,6 +1383,31 @@ hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const
   return hipGetLastError();
 }
 
+// Persistent grid: as many workgroups as the GPU holds at once (at most the
+// tile count), a multiple of the 8 XCDs so each keeps its tile order's XCD.
+template <typename T, int N, int AX, bool INV, int V>
+hipError_t launch_pipe(const PassArgs& A, long* nblocks_out, hipStream_t s) {
+  using C = TileCfg<T, N, AX, V>;
+  const long nt = tiles_of<T, N, AX, V>(A.geo);
+  if (nblocks_out) *nblocks_out = nt;
+  if (nt <= 0) return hipSuccess;
+  static const int resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pass_pipe<T, N, AX, INV, V>, C::BLOCK, 0) !=
+            hipSuccess)
+      return 0;
+    return cus * (per > 0 ? per : 1);
+  }();
+  if (resident <= 0) return hipErrorInvalidValue;
+  long nb = std::min<long>(nt, resident);
+  if (nb > 8) nb -= nb % 8;
+  hipLaunchKernelGGL((k_pass_pipe<T, N, AX, INV, V>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A,
+                     (int)nt);
+  return hipGetLastError();
+}
+
 // The (direction, load operator, store operator) combinations compiled for
 // one precision T, axis AX and length N.  Plain passes exist for every
 // compiled length; the fused operators exist where the step uses them:
@@ -1332,6 +1431,17 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
         if (variant == 3)
           return inv ? launch_one<T, N, AX, true, LD_PLAIN, ST_PLAIN, 3>(A, nb, s, B)
                      : launch_one<T, N, AX, false, LD_PLAIN, ST_PLAIN, 3>(A, nb, s, B);
+      }
+      return hipErrorInvalidValue;
+    }
+    if (variant == 4 || variant == 5) {
+      // pipelined forms of variants 0 and 2, single passes only
+      if (B || !inv) return hipErrorInvalidValue;
+      if constexpr (pipe_ok<T, N, AX, true, 4>()) {
+        if (variant == 4) return launch_pipe<T, N, AX, true, 4>(A, nb, s);
+      }
+      if constexpr (pipe_ok<T, N, AX, true, 5>()) {
+        if (variant == 5) return launch_pipe<T, N, AX, true, 5>(A, nb, s);
       }
       return hipErrorInvalidValue;
     }

@@ -83,6 +83,34 @@ def test_r2r_radix_variants(qsm_mod, oracle_mod, n, axis):
         assert rel_l2(got, ref) < 2e-6, (n, axis, v, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("n,prec,v", [(512, 0, 4), (512, 0, 5), (1024, 1, 4)])
+def test_r2r_pipelined(qsm_mod, oracle_mod, n, prec, v):
+    """The software-pipelined inverse x passes (variants 4 and 5: persistent
+    workgroups that load the next tile while transforming this one, where a
+    tile fills a CU's LDS alone) against the oracle, both inverse kinds along
+    x; forward kinds have no pipelined form."""
+    shape = (n, 8, 64)
+    rng = np.random.default_rng(n + v)
+    x = rng.standard_normal(shape)
+    for kind in KINDS:
+        kinds = [kind, 4, 5]
+        plan = qsm_mod.Plan(shape, kinds, prec)
+        if kind in (5, 9):  # REDFT10, RODFT10: forward
+            with pytest.raises(qsm_mod.QsmError) as ei:
+                plan.set_variant(0, v)
+            assert ei.value.code == -2
+            plan.close()
+            continue
+        plan.set_variant(0, v)
+        xin = x if prec == 0 else x.astype(np.float32)
+        got = plan(xin)
+        plan.close()
+        ref = oracle_mod.r2r_3d(np.asarray(xin, dtype=np.float64), kinds)
+        e = rel_l2(got, ref)
+        print(f"pipelined variant {v} n={n} prec {prec} kind {kind}: {e:.2e}", flush=True)
+        assert e < (1e-12 if prec == 0 else 2e-6), (n, v, kind, e)
+
+
 def test_r2r_variant_errors(qsm_mod):
     """A length without alternative plans refuses variants 2 and 3; a bad
     variant or axis is an argument error."""
@@ -93,8 +121,9 @@ def test_r2r_variant_errors(qsm_mod):
         assert ei.value.code == -2
         plan.set_variant(ax, 1)
         plan.set_variant(ax, 0)
-    with pytest.raises(qsm_mod.QsmError):
-        plan.set_variant(0, 4)
+    for bad in (4, 5, 6):  # no pipelined form where two tiles share a CU
+        with pytest.raises(qsm_mod.QsmError):
+            plan.set_variant(0, bad)
     with pytest.raises(qsm_mod.QsmError):
         plan.set_variant(3, 0)
     plan.close()
