@@ -11,6 +11,8 @@ against the oracle or a closed form:
   interface workload (a symmetric tanh blend at Lz/2);
 * the qsm_run driver's reference-format files hold the oracle's values at
   their printed precision (quasi.cpp:2008-2273, quasi-stb.cpp:1972-2158);
+* the focal-conic initial condition (quasi.cpp:543-589) gives the oracle's
+  run from an independent restatement of it;
 * resumes from the reference's psi<rank>.dat text (quasi.cpp:692-717) and
   from the binary checkpoint continue as the oracle does;
 * the 96-instance stability ensemble (500 steps): every 8th instance equals
@@ -24,6 +26,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import ic_ref
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -190,6 +193,34 @@ def test_driver_outputs_hold_oracle_values(qsm_mod, oracle_mod, tmp_path):
     for rk in range(4):
         _close(np.loadtxt(out / f"psi{rk}.dat"), psi[8 * rk:8 * (rk + 1)].reshape(-1), 1e-5,
                f"psi{rk}.dat")
+
+
+def test_focal_conic_driver_matches_oracle(qsm_mod, oracle_mod, tmp_path):
+    """qsm_run ic=focal (the focal-conic domain of quasi.cpp:543-589, size
+    fc = argv[3]) gives the run the oracle gives from the numpy restatement
+    of that initial condition: L1, mass and energy every step and the
+    mid-plane psi after 2 steps, at the files' 6 significant digits."""
+    shape, fc = (32, 32, 32), 12
+    d = _interface(shape, compat_ranks=0)
+    out = tmp_path / "focal"
+    keys = ["nx=32", "ny=32", "nz=32", "eps=-0.5", "beta=2", "gamma=1", "kp=0.2", "rho0=0.9",
+            "amp=0.5", "rho_m=0.95", "dt=0.01", "nu=1", "compat_ranks=0", "divv_switch=200",
+            "ic=focal", f"fc={fc}", "steps=2", "step_l1=1", "step_save=2", f"out={out}"]
+    r = subprocess.run([RUN] + keys, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    orc = oracle_mod.State(oracle_mod.make_params(**d))
+    orc.set_psi(ic_ref.focal_conic_psi(shape, d["amp"], d["q0"], d["nw"], fc))
+    L1, mass, energy = [], [], []
+    for _ in range(2):
+        orc.step(1)
+        dg = orc.diag()
+        L1.append(dg["L1"])
+        mass.append(dg["mass"])
+        energy.append(dg["energy"])
+    _close(np.loadtxt(out / "L1.dat"), L1, 1e-5, "L1.dat")
+    _close(np.loadtxt(out / "mass.dat"), mass, 1e-5, "mass.dat")
+    _close(np.loadtxt(out / "energy.dat"), energy, 1e-5, "energy.dat")
+    _close(np.loadtxt(out / "psiMid.dat"), _slice(orc.field("PSI"), 16), 1e-5, "psiMid.dat")
 
 
 def test_stb_driver_outputs(qsm_mod, oracle_mod, tmp_path):
