@@ -363,6 +363,9 @@ struct qsm_handle_s {
   bool divv_valid = false;    // the D field holds IDCT(scale D-hat) of the current step
   bool p_valid = false;       // the p field holds the current step's pressure
   bool r_valid = false;       // the grad rho fields hold the current step's (lagged) gradient
+  bool l_yspec = false;       // the lap grad psi fields hold their y-passed values (the stress
+                              // pass completed their inverse z passes in registers); the
+                              // real-space fields are formed on request (lapgrad_output)
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
   double* dsc = nullptr;
@@ -555,11 +558,13 @@ int run_group_first(qsm_handle_s* h, int group, const GroupItem* it, int count, 
 // inverse transforms x, y, z (the x pass as one group kernel).  The y and z
 // passes take the outputs last-written first: the group kernel's last tiles
 // are still in the memory-side cache when the next y pass starts on them.
-int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads) {
+// no_z bit o: output o stops after its y pass (its consumer runs the z pass)
+int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int reads,
+               unsigned no_z = 0) {
   int rc = run_group_first(h, group, it, count, reads);
   for (int o = count - 1; o >= 0 && !rc; --o) {
     rc = run_pass(h, 1, true, it[o].dst_axis == 1, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
-    if (!rc)
+    if (!rc && !((no_z >> o) & 1))
       rc = run_pass(h, 2, true, it[o].dst_axis == 2, plain_load(it[o].out), plain_store(it[o].out), false, nullptr);
   }
   return rc;
@@ -572,7 +577,8 @@ int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int r
 // inputs once (launch_multi), then y and x in place per value.  `reads`:
 // distinct fields the operator reads (algorithmic bytes).  Returns 1 (no
 // error set) where the pencil length has no multi-output kernel.
-int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int reads) {
+int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int reads,
+               int lz = 0) {
   if (!multi_supported(h->fp32, h->p.nz)) return 1;
   if (!h->dry) {
     if (h->rec) return fail(QSM_ERR_STATE, "multi-output passes cannot be recorded");
@@ -582,6 +588,7 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int
     A.ld = L;
     A.tw = h->tw[2];
     A.dmask = 1 << 2;  // component z: DST along z
+    A.lz = lz;
     for (int c = 0; c < nout; ++c) A.out[c] = out[c];
     Timer& T = h->timer;
     const int key = timer_key(2, false, L.code, ST_PLAIN, 0);
@@ -645,6 +652,11 @@ int quasi_step(qsm_handle_s* h) {
   h->r_valid = grad_rho_formed(p, n);
 
   const bool grp = !h->rec;  // group kernels (single handles)
+  // single handles form the stress components in one multi-output z pass,
+  // which also completes the inverse transforms of lap grad psi (their z
+  // passes are not run separately)
+  const bool multi = grp && multi_supported(h->fp32, p.nz);
+  h->l_yspec = multi;
   LoadOp L{};
   if (grp) {
     // S1 + S2 (quasi.cpp:1034-1232): F, G, grad psi, laplacian of grad psi
@@ -661,7 +673,7 @@ int quasi_step(qsm_handle_s* h) {
       it[5 + c].L.code = LD_LAPGRAD_X + c;
     }
     for (auto& x : it) { x.L.a[0] = psih; x.L.axis = x.dst_axis; }
-    if ((rc = inv3_group(h, GRP_PSI, it, 8, 1))) return rc;
+    if ((rc = inv3_group(h, GRP_PSI, it, 8, 1, multi ? 0xE0u : 0u))) return rc;
     if (dlive) {
       for (int c = 0; c < 3; ++c) {
         it[c] = {LoadOp{}, F(h, F_RX + c), c};
@@ -705,7 +717,6 @@ int quasi_step(qsm_handle_s* h) {
   // so the transform of zeros is not run.  S4 and S5 read only fields of
   // earlier stages: single handles run D (when live) as the fourth value of
   // the stress components' multi-output z pass.
-  const bool multi = grp && multi_supported(h->fp32, p.nz);
   if (dlive && !multi) {
     L = LoadOp{}; L.code = LD_DIVV;
     L.a[0] = rho; L.a[1] = rho_old; L.a[2] = F(h, F_VX); L.a[3] = F(h, F_VY); L.a[4] = F(h, F_VZ);
@@ -721,7 +732,7 @@ int quasi_step(qsm_handle_s* h) {
     L.a[9] = rho_old;
     for (int c = 0; c < 3; ++c) { L.a[10 + c] = F(h, F_VX + c); L.a[13 + c] = F(h, F_RX + c); }
     void* const out[4] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ), F(h, F_DHAT)};
-    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9))) return rc;
+    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9, 1))) return rc;
   } else {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
@@ -798,6 +809,20 @@ int pressure_output(qsm_handle_s* h) {
   const int rc = inv3(h, -1, pressure_load(h), F(h, F_P));
   if (!rc) h->p_valid = true;
   return rc;
+}
+
+// The lap grad psi fields of the current step in real space, formed on
+// request where the step left them y-passed (the stress pass ran their z
+// passes in registers): the same inverse z passes, in place.
+int lapgrad_output(qsm_handle_s* h) {
+  if (!h->l_yspec) return QSM_OK;
+  for (int c = 0; c < 3; ++c) {
+    void* f = F(h, F_LX + c);
+    const int rc = run_pass(h, 2, true, c == 2, plain_load(f), plain_store(f), false, nullptr);
+    if (rc) return rc;
+  }
+  h->l_yspec = false;
+  return QSM_OK;
 }
 
 // stb: dtpsi = -rho mu - v.g (hydro = 1) of the last step, rho = 1, mu =
@@ -1081,6 +1106,7 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   h->divv_valid = false;
   h->p_valid = false;
   h->r_valid = true;
+  h->l_yspec = false;
   h->ready = true;
   return QSM_OK;
 }
@@ -1243,7 +1269,12 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
                     h->step);
       src = F(h, F_RX + (which - QSM_RX));
       break;
-    case QSM_LX: case QSM_LY: case QSM_LZ: src = F(h, F_LX + (which - QSM_LX)); break;
+    case QSM_LX: case QSM_LY: case QSM_LZ: {
+      const int rc = lapgrad_output(h);
+      if (rc) return rc;
+      src = F(h, F_LX + (which - QSM_LX));
+      break;
+    }
     case QSM_RHOHAT: src = F(h, F_RHOHAT); break;
     case QSM_NHAT: src = F(h, F_NHAT_OLD); break;
     case QSM_SX: case QSM_SY: case QSM_SZ: src = F(h, F_SX + (which - QSM_SX)); break;
@@ -1810,6 +1841,9 @@ struct CkptHeader {
 extern "C" int qsm_checkpoint_write(qsm_handle h, const char* path) {
   if (!h || !path) return fail(QSM_ERR_ARG, "qsm_checkpoint_write: null argument");
   if (!h->ready) return fail(QSM_ERR_STATE, "qsm_checkpoint_write: no state");
+  // every field in real space (the reader takes them so)
+  const int lrc = lapgrad_output(h);
+  if (lrc) return lrc;
   HIPCHK(hipStreamSynchronize(h->stream));
   FILE* fp = fopen(path, "wb");
   if (!fp) return fail(QSM_ERR_ARG, "qsm_checkpoint_write: cannot open %s", path);
@@ -1866,6 +1900,7 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   h->divv_valid = false;
   h->p_valid = false;
   h->r_valid = grad_rho_formed(h->p, h->step);
+  h->l_yspec = false;  // written in real space (qsm_checkpoint_write)
   h->ready = true;
   if (h->p.variant == QSM_STB) return stb_growth(h);
   return QSM_OK;

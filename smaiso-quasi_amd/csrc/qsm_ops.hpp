@@ -497,15 +497,18 @@ template <int LD>
 constexpr int multi_count() {
   return LD == LD_STRESS3_DIVV ? 4 : ((LD == LD_STRESS3 || LD == LD_STRESS_STB3) ? 3 : 1);
 }
-template <typename T, int LD>
+// LZ: the lap grad psi components come from lz[0..2] (formed by the tile's
+// own inverse z round) instead of a[6..8].
+template <typename T, int LD, bool LZ = false>
 __device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx,
-                                           T (&out)[multi_count<LD>()]) {
+                                           T (&out)[multi_count<LD>()], const T* lz = nullptr) {
   if constexpr (LD == LD_STRESS3 || LD == LD_STRESS3_DIVV) {
     T in[5] = {ld<T>(L.a[0], idx), ld<T>(L.a[1], idx), ld<T>(L.a[2], idx), (T)0, (T)0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       in[3] = ld<T>(L.a[3 + c], idx);
-      in[4] = ld<T>(L.a[6 + c], idx);
+      if constexpr (LZ) in[4] = lz[c];
+      else in[4] = ld<T>(L.a[6 + c], idx);
       out[c] = op_apply<T, LD_STRESS>(in, P);
     }
     if constexpr (LD == LD_STRESS3_DIVV) {

@@ -283,6 +283,42 @@ constexpr int smem_tables() {
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
 using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<AX, LD, ST>()>()>;
 
+// DCT-III pre-processing of pencil p straight into the stage-1 inputs of
+// thread t (the real samples X of the pencil in LDS, natural order):
+// Z_m = (V_m + V_{m+M}) + i e^{2 pi i m/N} (V_m - V_{m+M}),
+// V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m}).
+// With a = X_m - i X_{N-m}, c = e^{i pi/4}(X_{M+m} - i X_{M-m}):
+// Z_m = A (a + c) + B (a - c),  A = e^{i pi m/2N},  B = i e^{2 pi i m/N} A = e^{i pi (5m+N)/2N}
+// (tp[2m], tp[2m+1]).
+template <typename T, int N, int M, int E, int TKP, int R1>
+__device__ __forceinline__ void inv_preprocess(cx<T>* v, const T* rbuf, const cx<T>* tp, int p,
+                                               int t) {
+  constexpr int TP = M / E, NB1 = E / R1;
+  const T h = (T)0.70710678118654752440;
+  // samples m, N-m, M+m, M-m of item m = t + C from per-thread bases
+  PencilPos<TKP> pp;
+  pp.init(p, t);
+#pragma unroll
+  for (int b = 0; b < NB1; ++b) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int C = b * TP + r * (M / R1);
+      const int m = t + C;
+      const T Xm = rbuf[pp.up(C)], XMp = rbuf[pp.up(M + C)], XMm = rbuf[pp.dn(M - C)];
+      T XNm;  // X_N = 0
+      if (C == 0) XNm = t == 0 ? (T)0 : rbuf[t == 0 ? pp.up(0) : pp.dn(N)];
+      else XNm = rbuf[pp.dn(N - C)];
+      // c = h (d1 + i d2)
+      const T d1 = XMp + XMm, d2 = XMp - XMm;
+      const cx<T> P = {fma(h, d1, Xm), fma(h, d2, -XNm)};
+      const cx<T> Q = {fma(-h, d1, Xm), fma(-h, d2, -XNm)};
+      const cx<T> A = tp[2 * m], B = tp[2 * m + 1];
+      v[b * R1 + r] = {fma(A.x, P.x, fma(-A.y, P.y, fma(B.x, Q.x, -B.y * Q.y))),
+                       fma(A.x, P.y, fma(A.y, P.x, fma(B.x, Q.y, B.y * Q.x)))};
+    }
+  }
+}
+
 // Values per thread of a pass's load walk (the tile's elements over the block).
 template <typename T, int N, int AX, int V>
 constexpr int walk_count() {
@@ -321,7 +357,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   using C = TileCfg<T, N, AX, V>;
   using PL = PlanV<T, N / 2, V>;
   constexpr int M = C::M, E = C::E, TP = C::TP, TK = C::TK, TKP = C::TKP, BLOCK = C::BLOCK;
-  constexpr int R1 = PL::R1, NB1 = E / R1;
+  constexpr int R1 = PL::R1;
   using TW = TwLayout<M>;
   constexpr int NTP = INV ? TW::NPRE : TW::NPOST;
   static_assert(SM::NTP == NTP && SM::TKP == TKP, "LDS layout of another pass");
@@ -683,35 +719,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
     }
     __syncthreads();
     if constexpr (FROMH) prefetch();
-    // ---- DCT-III pre-processing straight into the stage-1 inputs:
-    // Z_m = (V_m + V_{m+M}) + i e^{2 pi i m/N} (V_m - V_{m+M}),
-    // V_m = e^{i pi m/2N}(X_m - i X_{N-m}),  V_{m+M} = e^{i pi m/2N} e^{i pi/4}(X_{M+m} - i X_{M-m}).
-    // With a = X_m - i X_{N-m}, c = e^{i pi/4}(X_{M+m} - i X_{M-m}):
-    // Z_m = A (a + c) + B (a - c),  A = e^{i pi m/2N},  B = i e^{2 pi i m/N} A = e^{i pi (5m+N)/2N}
-    // (tp[2m], tp[2m+1]).
-    const T h = (T)0.70710678118654752440;
-    // samples m, N-m, M+m, M-m of item m = t + C from per-thread bases
-    PencilPos<TKP> pp;
-    pp.init(p, t);
-#pragma unroll
-    for (int b = 0; b < NB1; ++b) {
-#pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        const int C = b * TP + r * (M / R1);
-        const int m = t + C;
-        const T Xm = rbuf[pp.up(C)], XMp = rbuf[pp.up(M + C)], XMm = rbuf[pp.dn(M - C)];
-        T XNm;  // X_N = 0
-        if (C == 0) XNm = t == 0 ? (T)0 : rbuf[t == 0 ? pp.up(0) : pp.dn(N)];
-        else XNm = rbuf[pp.dn(N - C)];
-        // c = h (d1 + i d2)
-        const T d1 = XMp + XMm, d2 = XMp - XMm;
-        const cx<T> P = {fma(h, d1, Xm), fma(h, d2, -XNm)};
-        const cx<T> Q = {fma(-h, d1, Xm), fma(-h, d2, -XNm)};
-        const cx<T> A = tp[2 * m], B = tp[2 * m + 1];
-        v[b * R1 + r] = {fma(A.x, P.x, fma(-A.y, P.y, fma(B.x, Q.x, -B.y * Q.y))),
-                         fma(A.x, P.y, fma(A.y, P.x, fma(B.x, Q.y, B.y * Q.x)))};
-      }
-    }
+    inv_preprocess<T, N, M, E, TKP, R1>(v, rbuf, tp, p, t);
     if constexpr (AX == kAxZ) {
       run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, v, buf, p, t, ts);
       constexpr int PER = TK * N / BLOCK;
@@ -813,7 +821,15 @@ __host__ __device__ inline long multi_tiles(const Geo& g) {
   return ((long)g.nx * g.ny + R - 1) / R;
 }
 
-template <typename T, int N, int LD, class SM>
+// LZ (stress operators): the lap grad psi inputs a[6..8] hold their y-passed
+// values (the inverse transforms of lap grad psi stop before z).  The tile
+// first runs their inverse z pass itself: l_x, l_y, l_z go to pencil groups
+// 0-2 (group 3 is zero), one inverse transform runs on all TK pencils (DST-III
+// for l_z, by reversed positions and signs as in pass_tile), and each thread
+// keeps the real-space values of its walk elements in registers for the
+// operator.  The three inverse z passes of the separate transforms, a read
+// and a write of three fields, are then not run.
+template <typename T, int N, int LD, bool LZ, class SM>
 __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int tile, SM& sm,
                                                 const int tid) {
   using C = TileCfg<T, N, kAxZ, 0>;
@@ -828,11 +844,14 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   cx<T>* const tp = sm.tp;
   T* rbuf = (T*)buf;
 
-  {
-    const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
-    for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
+  static_assert(!LZ || LD == LD_STRESS3 || LD == LD_STRESS3_DIVV, "lap grad psi inputs");
+  static_assert(SM::NTP == (LZ ? TW::NPRE : TW::NPOST), "LDS layout");
+  const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
+  for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
+  if constexpr (LZ)
+    for (int u = tid; u < TW::NPRE; u += BLOCK) tp[u] = tw[TW::PRE + u];
+  else
     for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
-  }
   // the walk over the tile's R rows (element s of a thread: row q, sample n)
   TileGeo<kAxZ, N, R, BLOCK> tg;
   tg.init(A.geo, tile, tid);
@@ -840,16 +859,60 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   constexpr bool SPLIT = N % BLOCK == 0;
   constexpr int RZ = SPLIT ? N / BLOCK : 1;
   auto row_ok = [&](int s) { return tg.row0 + s / RZ < tg.nrows; };
+  auto walk_idx = [&](int s) {
+    int q, n, i, j, k;
+    Ix idx;
+    bool ok;
+    tg.walk(tid, s, q, n, i, j, k, idx, ok);
+    if constexpr (SPLIT) idx = tg.wbase + (Ix)(row_ok(s) ? s * BLOCK : (s % RZ) * BLOCK);
+    return idx;
+  };
+  const int p = tid % TK, t = tid / TK;
+  T lz[LZ ? PER : 1][3];  // real-space lap grad psi at the walk elements
+  if constexpr (LZ) {
+    {
+      WalkPos<kAxZ, N, R, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
+      WalkPos<kAxZ, N, R, TKP, BLOCK, 2> wr;  // reversed (DST-III: l_z)
+      wn.init(tid);
+      wr.init(tid);
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        const Ix idx = walk_idx(s);
+        T x[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = ld<T>(A.ld.a[6 + c], idx);
+        bool oddn;
+        const int pn = wn.pos(tid, s, oddn), pr = wr.pos(tid, s, oddn);
+        rbuf[pn] = x[0];
+        rbuf[pn + 2 * R] = x[1];
+        rbuf[pr + 4 * R] = x[2];
+        rbuf[pn + 6 * R] = (T)0;
+      }
+    }
+    __syncthreads();
+    cx<T> v[E];
+    inv_preprocess<T, N, M, E, TKP, R1>(v, rbuf, tp, p, t);
+    run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, v, buf, p, t, ts);
+    WalkPos<kAxZ, N, R, TKP, BLOCK, 1> wq;
+    wq.init(tid);
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      bool oddn;
+      const int lp = wq.pos(tid, s, oddn);
+      lz[s][0] = rbuf[lp];
+      lz[s][1] = rbuf[lp + 2 * R];
+      const T y = rbuf[lp + 4 * R];
+      lz[s][2] = oddn ? -y : y;
+    }
+    __syncthreads();  // the forward transform rewrites the tile and tp
+    for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
+  }
   {
     T xs[PER][NO];  // NO values of each element
 #pragma unroll
     for (int s = 0; s < PER; ++s) {
-      int q, n, i, j, k;
-      Ix idx;
-      bool ok;
-      tg.walk(tid, s, q, n, i, j, k, idx, ok);
-      if constexpr (SPLIT) idx = tg.wbase + (Ix)(row_ok(s) ? s * BLOCK : (s % RZ) * BLOCK);
-      multi_eval<T, LD>(A.ld, A.phys, idx, xs[s]);
+      const Ix idx = walk_idx(s);
+      multi_eval<T, LD, LZ>(A.ld, A.phys, idx, xs[s], lz[LZ ? s : 0]);
     }
     // Makhoul order; component o of row q in pencil o R + q
     WalkPos<kAxZ, N, R, TKP, BLOCK, 1> wp;
@@ -867,7 +930,6 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
     }
   }
   __syncthreads();
-  const int p = tid % TK, t = tid / TK;
   const bool dst = (A.dmask >> (p / R)) & 1;  // this thread's pencil group
   cx<T> v[E];
   gather<T, M, E, TKP, R1>(v, buf, p, t);
@@ -931,11 +993,13 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   }
 }
 
-template <typename T, int N, int LD>
+template <typename T, int N, int LD, bool LZ>
 __global__ void __launch_bounds__((TileCfg<T, N, kAxZ, 0>::BLOCK)) k_multi(const MultiArgs A) {
-  __shared__ Smem<T, N, kAxZ, false, 0, 0> sm;
+  // LZ: the tile's inverse round needs the DCT-III pre-processing twiddles
+  // (2M), more than the forward post-processing's 3(M/2+1)
+  __shared__ Smem<T, N, kAxZ, LZ, 0, 0> sm;
   const int nb = (int)gridDim.x;
-  pass_tile_multi<T, N, LD>(A, nb - 1 - (int)blockIdx.x, sm, (int)threadIdx.x);
+  pass_tile_multi<T, N, LD, LZ>(A, nb - 1 - (int)blockIdx.x, sm, (int)threadIdx.x);
 }
 
 template <typename T, int N>
@@ -947,12 +1011,13 @@ hipError_t launch_multi_n(const MultiArgs& A, long* nblocks, hipStream_t s) {
     if (nblocks) *nblocks = nb;
     if (nb <= 0) return hipSuccess;
     constexpr int BLOCK = TileCfg<T, N, kAxZ, 0>::BLOCK;
-    if (A.ld.code == LD_STRESS3)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
-    else if (A.ld.code == LD_STRESS_STB3)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
-    else if (A.ld.code == LD_STRESS3_DIVV)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3_DIVV>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    // the quasi stress always takes lap grad psi y-passed (A.lz = 1)
+    if (A.ld.code == LD_STRESS3 && A.lz)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3, true>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS_STB3 && !A.lz)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3, false>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS3_DIVV && A.lz)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3_DIVV, true>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
     else
       return hipErrorInvalidValue;
     return hipGetLastError();
