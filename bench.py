@@ -93,8 +93,9 @@ def kernel_symbol(k, shape, fp32):
     if k["ld"] >= 40:  # x-pass group kernel
         return f"void qsm::k_group<{t}, {n_ax}, {k['ld'] - 40}>(qsm::GroupArgs)"
     if k["ld"] < len(LOAD_OPS) and LOAD_OPS[k["ld"]] in ("stress3", "stress_stb3", "stress3_divv"):
-        # the quasi stress passes complete lap grad psi's inverse z passes (LZ)
-        lz = "false" if LOAD_OPS[k["ld"]] == "stress_stb3" else "true"
+        # the quasi stress passes complete lap grad psi's inverse z passes
+        # (LZ 1), with the divergence grad rho's too (LZ 2)
+        lz = {"stress3": 1, "stress_stb3": 0, "stress3_divv": 2}[LOAD_OPS[k["ld"]]]
         return f"void qsm::k_multi<{t}, {n_ax}, {k['ld']}, {lz}>(qsm::MultiArgs)"
     if k["variant"] >= 4:  # software-pipelined persistent plain pass
         return (f"void qsm::k_pass_pipe<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "

@@ -366,6 +366,8 @@ struct qsm_handle_s {
   bool l_yspec = false;       // the lap grad psi fields hold their y-passed values (the stress
                               // pass completed their inverse z passes in registers); the
                               // real-space fields are formed on request (lapgrad_output)
+  bool r_yspec = false;       // the same for grad rho (stress and nonlinear passes)
+  bool p_yspec = false;       // the same for p (nonlinear pass)
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
   double* dsc = nullptr;
@@ -415,7 +417,7 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
     case LD_VEL0: case LD_VEL1: case LD_VEL2: rd = (L.a[3] ? 4 : 3) * e; break;
     case LD_SCALE: rd = e; break;
     case LD_NONLIN: rd = 10 * e; break;
-    case LD_NONLIN_P: rd = 14 * e; break;
+    case LD_NONLIN_P: case LD_NONLIN_PY: rd = 14 * e; break;
     case LD_STRESS_STB: rd = 3 * e; break;
     case LD_NONLIN_STB: rd = 7 * e; break;
     case LD_NONLIN_STB0: rd = e; break;
@@ -647,15 +649,17 @@ int quasi_step(qsm_handle_s* h) {
   // arrays hold from set-up, and p is formed when the field is asked for
   // (pressure_output).
   const bool plive = n > p.divv_switch + 1;
-  h->divv_valid = false;
-  h->p_valid = plive;
-  h->r_valid = grad_rho_formed(p, n);
-
   const bool grp = !h->rec;  // group kernels (single handles)
-  // single handles form the stress components in one multi-output z pass,
-  // which also completes the inverse transforms of lap grad psi (their z
-  // passes are not run separately)
+  // Single handles form the stress components (and D) in one multi-output z
+  // pass.  That pass completes the inverse transforms of lap grad psi and,
+  // once D is live, of grad rho; the nonlinear pass completes those of grad
+  // rho and p.  Their z passes are not run separately.
   const bool multi = grp && multi_supported(h->fp32, p.nz);
+  h->divv_valid = false;
+  h->p_valid = plive && !multi;
+  h->p_yspec = plive && multi;
+  h->r_valid = grad_rho_formed(p, n);
+  h->r_yspec = dlive && multi;
   h->l_yspec = multi;
   LoadOp L{};
   if (grp) {
@@ -681,7 +685,7 @@ int quasi_step(qsm_handle_s* h) {
         it[c].L.axis = c;
         it[c].L.a[0] = F(h, F_RHOHAT);
       }
-      if ((rc = inv3_group(h, GRP_RHO, it, 3, 1))) return rc;
+      if ((rc = inv3_group(h, GRP_RHO, it, 3, 1, multi ? 0x7u : 0u))) return rc;
     }
   } else {
     // S1: F, G (quasi.cpp:1034-1057)
@@ -732,7 +736,7 @@ int quasi_step(qsm_handle_s* h) {
     L.a[9] = rho_old;
     for (int c = 0; c < 3; ++c) { L.a[10 + c] = F(h, F_VX + c); L.a[13 + c] = F(h, F_RX + c); }
     void* const out[4] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ), F(h, F_DHAT)};
-    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9, 1))) return rc;
+    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9, dlive ? 2 : 1))) return rc;
   } else {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
@@ -754,7 +758,7 @@ int quasi_step(qsm_handle_s* h) {
       it[1 + c].L.a[3] = dlive ? F(h, F_DHAT) : nullptr;  // D-hat = 0 before: its term vanishes
     }
     for (auto& x : it) { x.L.a[0] = F(h, F_SX); x.L.a[1] = F(h, F_SY); x.L.a[2] = F(h, F_SZ); }
-    if (plive) rc = inv3_group(h, GRP_PROJ, it, 4, 4);
+    if (plive) rc = inv3_group(h, GRP_PROJ, it, 4, 4, multi ? 0x1u : 0u);
     else rc = inv3_group(h, GRP_VEL, it + 1, 3, dlive ? 4 : 3);
     if (rc) return rc;
   } else {
@@ -772,7 +776,7 @@ int quasi_step(qsm_handle_s* h) {
   // (divv_output).
 
   // S9 + S10: N, its transform, and the update fused into the last pass
-  L = LoadOp{}; L.code = plive ? LD_NONLIN_P : LD_NONLIN;
+  L = LoadOp{}; L.code = plive ? (multi ? LD_NONLIN_PY : LD_NONLIN_P) : LD_NONLIN;
   L.a[0] = rho; L.a[1] = psi; L.a[2] = F(h, F_F); L.a[3] = F(h, F_G);
   L.a[4] = F(h, F_VX); L.a[5] = F(h, F_VY); L.a[6] = F(h, F_VZ);
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
@@ -806,8 +810,15 @@ int divv_output(qsm_handle_s* h) {
 // same x pass as the projection group's first output, so the same bits.
 int pressure_output(qsm_handle_s* h) {
   if (h->p_valid) return QSM_OK;
-  const int rc = inv3(h, -1, pressure_load(h), F(h, F_P));
-  if (!rc) h->p_valid = true;
+  // y-passed by the step (its nonlinear pass completed the transform in
+  // registers): the z pass in place; else the whole transform from S-hat
+  const int rc = h->p_yspec
+                     ? run_pass(h, 2, true, false, plain_load(F(h, F_P)), plain_store(F(h, F_P)), false, nullptr)
+                     : inv3(h, -1, pressure_load(h), F(h, F_P));
+  if (!rc) {
+    h->p_valid = true;
+    h->p_yspec = false;
+  }
   return rc;
 }
 
@@ -822,6 +833,18 @@ int lapgrad_output(qsm_handle_s* h) {
     if (rc) return rc;
   }
   h->l_yspec = false;
+  return QSM_OK;
+}
+
+// The same for grad rho (component c: DST-III along c).
+int gradrho_output(qsm_handle_s* h) {
+  if (!h->r_yspec) return QSM_OK;
+  for (int c = 0; c < 3; ++c) {
+    void* f = F(h, F_RX + c);
+    const int rc = run_pass(h, 2, true, c == 2, plain_load(f), plain_store(f), false, nullptr);
+    if (rc) return rc;
+  }
+  h->r_yspec = false;
   return QSM_OK;
 }
 
@@ -1106,7 +1129,7 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   h->divv_valid = false;
   h->p_valid = false;
   h->r_valid = true;
-  h->l_yspec = false;
+  h->l_yspec = h->r_yspec = h->p_yspec = false;
   h->ready = true;
   return QSM_OK;
 }
@@ -1267,6 +1290,10 @@ extern "C" int qsm_get_field(qsm_handle h, int which, double* host) {
         return fail(QSM_ERR_STATE,
                     "qsm_get_field: grad rho is formed at step 1 and after divv_switch only (step %lld)",
                     h->step);
+      {
+        const int rc = gradrho_output(h);
+        if (rc) return rc;
+      }
       src = F(h, F_RX + (which - QSM_RX));
       break;
     case QSM_LX: case QSM_LY: case QSM_LZ: {
@@ -1842,7 +1869,9 @@ extern "C" int qsm_checkpoint_write(qsm_handle h, const char* path) {
   if (!h || !path) return fail(QSM_ERR_ARG, "qsm_checkpoint_write: null argument");
   if (!h->ready) return fail(QSM_ERR_STATE, "qsm_checkpoint_write: no state");
   // every field in real space (the reader takes them so)
-  const int lrc = lapgrad_output(h);
+  int lrc = lapgrad_output(h);
+  if (!lrc) lrc = gradrho_output(h);
+  if (!lrc && h->p_yspec) lrc = pressure_output(h);
   if (lrc) return lrc;
   HIPCHK(hipStreamSynchronize(h->stream));
   FILE* fp = fopen(path, "wb");
@@ -1900,7 +1929,7 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   h->divv_valid = false;
   h->p_valid = false;
   h->r_valid = grad_rho_formed(h->p, h->step);
-  h->l_yspec = false;  // written in real space (qsm_checkpoint_write)
+  h->l_yspec = h->r_yspec = h->p_yspec = false;  // written in real space (qsm_checkpoint_write)
   h->ready = true;
   if (h->p.variant == QSM_STB) return stb_growth(h);
   return QSM_OK;

@@ -92,8 +92,8 @@ struct GroupArgs {
 // divergence, from one read of their shared inputs): ld is a multi-output
 // operator (LD_STRESS3, LD_STRESS_STB3, LD_STRESS3_DIVV), out[c] receives
 // value c, dmask bit c selects DST along z for value c.  The quasi stress
-// operators take lap grad psi y-passed (lz = 1): the pass completes its
-// inverse transforms itself.
+// operators take lap grad psi y-passed (lz = 1), and with the divergence
+// grad rho too (lz = 2): the pass completes their inverse transforms itself.
 struct MultiArgs {
   Geo geo;
   Phys phys;
@@ -101,7 +101,8 @@ struct MultiArgs {
   void* out[4];
   const void* tw;  // z-axis twiddle table
   int dmask;
-  int lz;          // 1: a[6..8] (lap grad psi) are y-passed; the pass runs their inverse z round
+  int lz;          // 1: a[6..8] (lap grad psi) are y-passed, 2: a[13..15] (grad rho) too;
+                   // the pass runs their inverse z rounds
 };
 hipError_t launch_multi(int prec_fp32, int n, const MultiArgs& A, long* nblocks, hipStream_t s);
 bool multi_supported(int prec_fp32, int n);

@@ -66,6 +66,8 @@ enum LoadCode : int {
   LD_STRESS_STB3,   // LD_STRESS_STB for c = x, y, z; a: sh, psi, g_x,y,z
   LD_STRESS3_DIVV,  // LD_STRESS3, then LD_DIVV as a fourth value; a: as LD_STRESS3, then
                     //   rho_old, v_x,y,z, r_x,y,z (rho is a0)
+  LD_NONLIN_PY,     // LD_NONLIN_P with p (a10) and grad rho (a11-13) y-passed: the pass
+                    //   completes their inverse z transforms and keeps the pressure term
 };
 
 enum StoreCode : int {
@@ -356,7 +358,8 @@ __device__ __forceinline__ T finish(const Raw4<T>& r, const LoadOp& L, const Phy
 template <int LD>
 constexpr int op_nin() {
   return LD == LD_PLAIN ? 1 : LD == LD_RHOSTAR ? 4 : LD == LD_DIVV ? 8 : LD == LD_STRESS ? 5
-       : LD == LD_NONLIN ? 10 : LD == LD_NONLIN_P ? 14 : LD == LD_STRESS_STB ? 3
+       : LD == LD_NONLIN ? 10 : LD == LD_NONLIN_P ? 14 : LD == LD_NONLIN_PY ? 11
+       : LD == LD_STRESS_STB ? 3
        : LD == LD_NONLIN_STB ? 7 : LD == LD_NONLIN_STB0 ? 1 : 0;
 }
 template <typename T, int LD>
@@ -379,8 +382,9 @@ __device__ __forceinline__ T op_apply(const T* in, const Phys& P) {
     const T gg = r * in[1];
     const T rf = r * in[2];
     return gg * in[3] - rf * in[4];
-  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P) {
-    // rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz
+  } else if constexpr (LD == LD_NONLIN || LD == LD_NONLIN_P || LD == LD_NONLIN_PY) {
+    // rho, psi, F, G, vx, vy, vz, gx, gy, gz, p, rx, ry, rz; LD_NONLIN_PY:
+    // in[10] is the pressure term itself (pressure_term, formed by the pass)
     const T r = in[0], ps = in[1];
     const T ps2 = ps * ps, ps3 = ps2 * ps, ps5 = ps3 * ps2;
     T v = r * ((T)P.eps * ps - in[2] + (T)P.beta * ps3 - (T)P.gamma * ps5) - r * in[3];
@@ -390,6 +394,8 @@ __device__ __forceinline__ T op_apply(const T* in, const Phys& P) {
     if constexpr (LD == LD_NONLIN_P) {
       const T rx = in[11], ry = in[12], rz = in[13];
       v = v + in[10] * (T)P.kp * sqrt(rx * rx + ry * ry + rz * rz) / (T)P.rho_s2;
+    } else if constexpr (LD == LD_NONLIN_PY) {
+      v = v + in[10];
     }
     return v;
   } else if constexpr (LD == LD_STRESS_STB) {
@@ -416,9 +422,18 @@ __device__ __forceinline__ T op_apply(const T* in, const Phys& P) {
 }
 
 // K carries the element's wavenumbers for the x-pass spectral operators.
+// The pressure term p kp |grad rho| / rho_s^2 of quasi.cpp:1925-1930, from
+// |grad rho|^2 accumulated as rx^2, + ry^2, + rz^2 (the order of LD_NONLIN_P).
+template <typename T>
+__device__ __forceinline__ T pressure_term(T p, T r2, const Phys& P) {
+  return p * (T)P.kp * sqrt(r2) / (T)P.rho_s2;
+}
+
+// held: a value the pass formed itself (LD_NONLIN_PY: the pressure term).
 template <typename T, int LD>
 __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv& K,
-                                       const Geo& G, int i, int j, int k, Ix idx) {
+                                       const Geo& G, int i, int j, int k, Ix idx,
+                                       T held = (T)0) {
   constexpr bool MIX = sizeof(T) == sizeof(float);  // FP32 fields: mixed precision
   if constexpr (LD == LD_PLAIN) {
     return ld<T>(L.a[0], idx);
@@ -484,6 +499,12 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
 #pragma unroll
     for (int f = 0; f < op_nin<LD>(); ++f) in[f] = ld<T>(L.a[f], idx);
     return op_apply<T, LD>(in, P);
+  } else if constexpr (LD == LD_NONLIN_PY) {
+    T in[11];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) in[f] = ld<T>(L.a[f], idx);
+    in[10] = held;
+    return op_apply<T, LD>(in, P);
   } else {
     static_assert(LD == LD_PLAIN, "unknown load operator");
     return (T)0;
@@ -497,9 +518,10 @@ template <int LD>
 constexpr int multi_count() {
   return LD == LD_STRESS3_DIVV ? 4 : ((LD == LD_STRESS3 || LD == LD_STRESS_STB3) ? 3 : 1);
 }
-// LZ: the lap grad psi components come from lz[0..2] (formed by the tile's
-// own inverse z round) instead of a[6..8].
-template <typename T, int LD, bool LZ = false>
+// LZ >= 1: the lap grad psi components come from lz[0..2] (formed by the
+// tile's own inverse z round) instead of a[6..8]; LZ = 2: grad rho as well,
+// from lz[3..5] instead of a[13..15].
+template <typename T, int LD, int LZ = 0>
 __device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix idx,
                                            T (&out)[multi_count<LD>()], const T* lz = nullptr) {
   if constexpr (LD == LD_STRESS3 || LD == LD_STRESS3_DIVV) {
@@ -507,14 +529,17 @@ __device__ __forceinline__ void multi_eval(const LoadOp& L, const Phys& P, Ix id
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       in[3] = ld<T>(L.a[3 + c], idx);
-      if constexpr (LZ) in[4] = lz[c];
+      if constexpr (LZ >= 1) in[4] = lz[c];
       else in[4] = ld<T>(L.a[6 + c], idx);
       out[c] = op_apply<T, LD_STRESS>(in, P);
     }
     if constexpr (LD == LD_STRESS3_DIVV) {
       T d[8] = {in[0]};
 #pragma unroll
-      for (int f = 1; f < 8; ++f) d[f] = ld<T>(L.a[8 + f], idx);
+      for (int f = 1; f < 8; ++f) {
+        if constexpr (LZ == 2) d[f] = f >= 5 ? lz[f - 2] : ld<T>(L.a[8 + f], idx);
+        else d[f] = ld<T>(L.a[8 + f], idx);
+      }
       out[3] = op_apply<T, LD_DIVV>(d, P);
     }
   } else {
@@ -603,9 +628,14 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
 // mixed precision (the density law and the nonlinear terms): their forward
 // z pass accumulates an FP64 sum of the loaded values per workgroup.
 __host__ __device__ constexpr bool dc_capable(int ld) {
-  return ld == LD_RHOSTAR || ld == LD_NONLIN || ld == LD_NONLIN_P || ld == LD_NONLIN_STB ||
-         ld == LD_NONLIN_STB0;
+  return ld == LD_RHOSTAR || ld == LD_NONLIN || ld == LD_NONLIN_P || ld == LD_NONLIN_PY ||
+         ld == LD_NONLIN_STB || ld == LD_NONLIN_STB0;
 }
+
+// Forward z passes that first complete the inverse z transforms of some of
+// their inputs (y-passed), holding the result in registers: they need the
+// DCT-III pre-processing twiddles as well.
+__host__ __device__ constexpr bool pre_rounds(int ld) { return ld == LD_NONLIN_PY; }
 
 // Load operators that read many fields or do FP64 projection arithmetic are
 // evaluated in chunks of 8 elements (register pressure); the rest in one go.

@@ -280,8 +280,12 @@ constexpr int smem_tables() {
   return (NT != 0 && Smem<T, N, AX, INV, V, NT>::bytes() <= 81920) ? NT : 0;
 }
 
+// Passes with inverse pre-rounds (pre_rounds) stage the DCT-III
+// pre-processing twiddles too: their LDS takes the inverse layout (2M
+// entries, more than the forward post-processing's 3(M/2+1)).
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
-using SmemFor = Smem<T, N, AX, INV, V, smem_tables<T, N, AX, INV, V, kv_tables<AX, LD, ST>()>()>;
+using SmemFor = Smem<T, N, AX, INV || pre_rounds(LD), V,
+                     smem_tables<T, N, AX, INV || pre_rounds(LD), V, kv_tables<AX, LD, ST>()>()>;
 
 // DCT-III pre-processing of pencil p straight into the stage-1 inputs of
 // thread t (the real samples X of the pencil in LDS, natural order):
@@ -360,7 +364,9 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   constexpr int R1 = PL::R1;
   using TW = TwLayout<M>;
   constexpr int NTP = INV ? TW::NPRE : TW::NPOST;
-  static_assert(SM::NTP == NTP && SM::TKP == TKP, "LDS layout of another pass");
+  static_assert(SM::NTP == (INV || pre_rounds(LD) ? TW::NPRE : NTP) && SM::TKP == TKP,
+                "LDS layout of another pass");
+  static_assert(!pre_rounds(LD) || (AX == kAxZ && !INV), "pre-rounds: forward z passes");
   cx<T>* const buf = sm.buf;
   cx<T>* const ts = sm.ts;
   cx<T>* const tp = sm.tp;
@@ -398,7 +404,10 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   if (stage) {
     const cx<T>* __restrict__ tw = (const cx<T>*)twp;
     for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
-    for (int u = tid; u < NTP; u += BLOCK) tp[u] = tw[(INV ? TW::PRE : TW::POST) + u];
+    if constexpr (pre_rounds(LD))  // the pre-rounds first; the forward's tables after them
+      for (int u = tid; u < TW::NPRE; u += BLOCK) tp[u] = tw[TW::PRE + u];
+    else
+      for (int u = tid; u < NTP; u += BLOCK) tp[u] = tw[(INV ? TW::PRE : TW::POST) + u];
     if constexpr (SM::ntab > 0) {
       for (int u = tid; u < N; u += BLOCK) {
         if constexpr ((SM::tab & 1) != 0) sm.xt[SQ * N + u] = tq[u];
@@ -472,6 +481,55 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   double dsum = 0.0;
   cx<T> v[E];
 
+  // LD_NONLIN_PY: p (a10) and grad rho (a11-13) arrive y-passed.  Four
+  // inverse z rounds over the tile (r_x, r_y, r_z with DST-III along z,
+  // then p) complete their transforms; each thread folds the values of its
+  // walk elements into the pressure term p kp |grad rho| / rho_s^2, the one
+  // value the operator then takes from registers.  Their separate inverse z
+  // passes are not run.
+  T pterm[pre_rounds(LD) ? TK * N / BLOCK : 1];
+  if constexpr (pre_rounds(LD)) {
+    constexpr int PER = TK * N / BLOCK;
+    WalkPos<AX, N, TK, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
+    WalkPos<AX, N, TK, TKP, BLOCK, 2> wr;  // reversed (DST-III)
+    WalkPos<AX, N, TK, TKP, BLOCK, 1> wq;  // the transform's output positions
+    wn.init(tid);
+    wr.init(tid);
+    wq.init(tid);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const void* src = Lop.a[c < 3 ? 11 + c : 10];
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        int q, n, i, j, k;
+        Ix idx;
+        bool ok, oddn;
+        tg.walk(tid, s, q, n, i, j, k, idx, ok);
+        T x;
+        if constexpr (SPLIT) x = ld_at<T>(src, tg.wbase, walk_uni(s));
+        else x = ld<T>(src, idx);
+        rbuf[c == 2 ? wr.pos(tid, s, oddn) : wn.pos(tid, s, oddn)] = x;
+      }
+      __syncthreads();
+      cx<T> vr[E];
+      inv_preprocess<T, N, M, E, TKP, R1>(vr, rbuf, tp, p, t);
+      run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, vr, buf, p, t, ts);
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        bool oddn;
+        T y = rbuf[wq.pos(tid, s, oddn)];
+        if (c == 2 && oddn) y = -y;
+        if (c == 0) pterm[s] = y * y;
+        else if (c < 3) pterm[s] = pterm[s] + y * y;
+        else pterm[s] = pressure_term<T>(y, pterm[s], Ph);
+      }
+      __syncthreads();  // the next round (or the load phase) rewrites the tile
+    }
+    // the forward post-processing twiddles, read after the load phase's barrier
+    const cx<T>* __restrict__ tw = (const cx<T>*)twp;
+    for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
+  }
+
   if constexpr (!INV) {
     {
       // row-sweeping loads into LDS in Makhoul order (a wavefront reads 64
@@ -481,6 +539,7 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
       constexpr int PER = TK * N / BLOCK;
       constexpr int CH = FROMH ? PER : (load_heavy<LD>() ? (PER < 8 ? PER : 8) : PER);
       static_assert(!FROMH || PER == walk_count<T, N, AX, V>(), "walk length");
+      static_assert(!pre_rounds(LD) || CH == PER, "held values: one chunk");
       WalkPos<AX, N, TK, TKP, BLOCK, 1> wp;
       wp.init(tid);
 #pragma unroll 1
@@ -494,6 +553,8 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
           tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
           if constexpr (FROMH) xs[c] = held[c];  // one chunk (CH = PER): s0 = 0
           else if constexpr (LD == LD_PLAIN && SPLIT) xs[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c));
+          else if constexpr (pre_rounds(LD))  // one chunk (CH = PER): s0 = 0
+            xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx, pterm[c]);
           else xs[c] = load_eval<T, LD>(Lop, Ph, kv_at(i, j, k, q), G, i, j, k, idx);
         }
         // values of rows or columns outside the field (read from a valid
@@ -821,15 +882,71 @@ __host__ __device__ inline long multi_tiles(const Geo& g) {
   return ((long)g.nx * g.ny + R - 1) / R;
 }
 
-// LZ (stress operators): the lap grad psi inputs a[6..8] hold their y-passed
-// values (the inverse transforms of lap grad psi stop before z).  The tile
-// first runs their inverse z pass itself: l_x, l_y, l_z go to pencil groups
-// 0-2 (group 3 is zero), one inverse transform runs on all TK pencils (DST-III
+// One inverse z round of a multi-output tile: field a[F_g] (F_g < 0: zero)
+// in pencil group g, DST-III for group DG; each thread keeps the outputs of
+// its walk elements, group g at lz[s][BASE + g].  Every load of the round is
+// issued before the first LDS write, so they are in flight together.
+template <typename T, int N, int R, int TKP, int BLOCK, int M, int E, int R1, class RADICES>
+struct MultiRound {
+  static constexpr int PER = R * N / BLOCK;
+  template <int F0, int F1, int F2, int F3, int DG, int BASE, class WI>
+  __device__ __forceinline__ static void run(const LoadOp& L, const WI& walk_idx, T* rbuf,
+                                             cx<T>* buf, const cx<T>* tp, const cx<T>* ts,
+                                             int p, int t, int tid, T (*lz)[6]) {
+    constexpr int F[4] = {F0, F1, F2, F3};
+    {
+      T x[PER][4];
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        const Ix idx = walk_idx(s);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) x[s][g] = F[g] >= 0 ? ld<T>(L.a[F[g] >= 0 ? F[g] : 0], idx) : (T)0;
+      }
+      WalkPos<kAxZ, N, R, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
+      WalkPos<kAxZ, N, R, TKP, BLOCK, 2> wr;  // reversed (DST-III)
+      wn.init(tid);
+      wr.init(tid);
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        bool oddn;
+        const int pn = wn.pos(tid, s, oddn), pr = wr.pos(tid, s, oddn);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) rbuf[(g == DG ? pr : pn) + 2 * g * R] = x[s][g];
+      }
+    }
+    __syncthreads();
+    cx<T> v[E];
+    inv_preprocess<T, N, M, E, TKP, R1>(v, rbuf, tp, p, t);
+    run_fft<T, N, M, E, TKP, 1, true>(RADICES{}, v, buf, p, t, ts);
+    WalkPos<kAxZ, N, R, TKP, BLOCK, 1> wq;
+    wq.init(tid);
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      bool oddn;
+      const int lp = wq.pos(tid, s, oddn);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (F[g] >= 0) {  // constant after unrolling; the index guard keeps unused groups in bounds
+          const T y = rbuf[lp + 2 * g * R];
+          lz[s][BASE + g < 6 ? BASE + g : 5] = (g == DG && oddn) ? -y : y;
+        }
+      }
+    }
+    __syncthreads();  // the next round or the forward transform rewrites the tile
+  }
+};
+
+// LZ (stress operators) >= 1: the lap grad psi inputs a[6..8] hold their
+// y-passed values (the inverse transforms of lap grad psi stop before z).
+// The tile first runs their inverse z pass itself: l_x, l_y, l_z go to
+// pencil groups 0-2, one inverse transform runs on all TK pencils (DST-III
 // for l_z, by reversed positions and signs as in pass_tile), and each thread
 // keeps the real-space values of its walk elements in registers for the
-// operator.  The three inverse z passes of the separate transforms, a read
-// and a write of three fields, are then not run.
-template <typename T, int N, int LD, bool LZ, class SM>
+// operator.  LZ = 2 (with the divergence): grad rho a[13..15] as well, r_x in
+// group 3 of that round and r_y, r_z (DST-III) in groups 0-1 of a second.
+// Unused groups are zero.  The separate inverse z passes, a read and a write
+// of each field, are then not run.
+template <typename T, int N, int LD, int LZ, class SM>
 __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int tile, SM& sm,
                                                 const int tid) {
   using C = TileCfg<T, N, kAxZ, 0>;
@@ -844,11 +961,12 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   cx<T>* const tp = sm.tp;
   T* rbuf = (T*)buf;
 
-  static_assert(!LZ || LD == LD_STRESS3 || LD == LD_STRESS3_DIVV, "lap grad psi inputs");
+  static_assert(LZ == 0 || LD == LD_STRESS3 || LD == LD_STRESS3_DIVV, "lap grad psi inputs");
+  static_assert(LZ < 2 || LD == LD_STRESS3_DIVV, "grad rho inputs");
   static_assert(SM::NTP == (LZ ? TW::NPRE : TW::NPOST), "LDS layout");
   const cx<T>* __restrict__ tw = (const cx<T>*)A.tw;
   for (int u = tid; u < M; u += BLOCK) ts[u] = tw[TW::STAGE + u];
-  if constexpr (LZ)
+  if constexpr (LZ != 0)
     for (int u = tid; u < TW::NPRE; u += BLOCK) tp[u] = tw[TW::PRE + u];
   else
     for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
@@ -868,43 +986,13 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
     return idx;
   };
   const int p = tid % TK, t = tid / TK;
-  T lz[LZ ? PER : 1][3];  // real-space lap grad psi at the walk elements
-  if constexpr (LZ) {
-    {
-      WalkPos<kAxZ, N, R, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
-      WalkPos<kAxZ, N, R, TKP, BLOCK, 2> wr;  // reversed (DST-III: l_z)
-      wn.init(tid);
-      wr.init(tid);
-#pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        const Ix idx = walk_idx(s);
-        T x[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) x[c] = ld<T>(A.ld.a[6 + c], idx);
-        bool oddn;
-        const int pn = wn.pos(tid, s, oddn), pr = wr.pos(tid, s, oddn);
-        rbuf[pn] = x[0];
-        rbuf[pn + 2 * R] = x[1];
-        rbuf[pr + 4 * R] = x[2];
-        rbuf[pn + 6 * R] = (T)0;
-      }
-    }
-    __syncthreads();
-    cx<T> v[E];
-    inv_preprocess<T, N, M, E, TKP, R1>(v, rbuf, tp, p, t);
-    run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, v, buf, p, t, ts);
-    WalkPos<kAxZ, N, R, TKP, BLOCK, 1> wq;
-    wq.init(tid);
-#pragma unroll
-    for (int s = 0; s < PER; ++s) {
-      bool oddn;
-      const int lp = wq.pos(tid, s, oddn);
-      lz[s][0] = rbuf[lp];
-      lz[s][1] = rbuf[lp + 2 * R];
-      const T y = rbuf[lp + 4 * R];
-      lz[s][2] = oddn ? -y : y;
-    }
-    __syncthreads();  // the forward transform rewrites the tile and tp
+  // real-space inputs at the walk elements: l_x, l_y, l_z, r_x, r_y, r_z
+  T lz[LZ ? PER : 1][6];
+  if constexpr (LZ != 0) {
+    using IR = MultiRound<T, N, R, TKP, BLOCK, M, E, R1, typename PL::R>;
+    IR::template run<6, 7, 8, (LZ == 2 ? 13 : -1), 2, 0>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t,
+                                                        tid, lz);
+    if constexpr (LZ == 2) IR::template run<14, 15, -1, -1, 1, 4>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t, tid, lz);
     for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
   }
   {
@@ -993,11 +1081,11 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   }
 }
 
-template <typename T, int N, int LD, bool LZ>
+template <typename T, int N, int LD, int LZ>
 __global__ void __launch_bounds__((TileCfg<T, N, kAxZ, 0>::BLOCK)) k_multi(const MultiArgs A) {
-  // LZ: the tile's inverse round needs the DCT-III pre-processing twiddles
+  // LZ: the tile's inverse rounds need the DCT-III pre-processing twiddles
   // (2M), more than the forward post-processing's 3(M/2+1)
-  __shared__ Smem<T, N, kAxZ, LZ, 0, 0> sm;
+  __shared__ Smem<T, N, kAxZ, (LZ != 0), 0, 0> sm;
   const int nb = (int)gridDim.x;
   pass_tile_multi<T, N, LD, LZ>(A, nb - 1 - (int)blockIdx.x, sm, (int)threadIdx.x);
 }
@@ -1011,13 +1099,14 @@ hipError_t launch_multi_n(const MultiArgs& A, long* nblocks, hipStream_t s) {
     if (nblocks) *nblocks = nb;
     if (nb <= 0) return hipSuccess;
     constexpr int BLOCK = TileCfg<T, N, kAxZ, 0>::BLOCK;
-    // the quasi stress always takes lap grad psi y-passed (A.lz = 1)
-    if (A.ld.code == LD_STRESS3 && A.lz)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3, true>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
-    else if (A.ld.code == LD_STRESS_STB3 && !A.lz)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3, false>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
-    else if (A.ld.code == LD_STRESS3_DIVV && A.lz)
-      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3_DIVV, true>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    // the quasi stress takes lap grad psi y-passed (A.lz = 1), and with the
+    // divergence grad rho as well (A.lz = 2)
+    if (A.ld.code == LD_STRESS3 && A.lz == 1)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3, 1>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS_STB3 && A.lz == 0)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS_STB3, 0>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
+    else if (A.ld.code == LD_STRESS3_DIVV && A.lz == 2)
+      hipLaunchKernelGGL((k_multi<T, N, LD_STRESS3_DIVV, 2>), dim3((unsigned)nb), dim3(BLOCK), 0, s, A);
     else
       return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1518,7 +1607,8 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
       switch (ld) {
 #define QSM_L(OP) case OP: return launch_one<T, N, AX, false, OP, ST_PLAIN>(A, nb, s, B);
         QSM_L(LD_ZERO) QSM_L(LD_RHOSTAR) QSM_L(LD_DIVV) QSM_L(LD_STRESS) QSM_L(LD_NONLIN)
-        QSM_L(LD_NONLIN_P) QSM_L(LD_STRESS_STB) QSM_L(LD_NONLIN_STB) QSM_L(LD_NONLIN_STB0)
+        QSM_L(LD_NONLIN_P) QSM_L(LD_NONLIN_PY) QSM_L(LD_STRESS_STB) QSM_L(LD_NONLIN_STB)
+        QSM_L(LD_NONLIN_STB0)
 #undef QSM_L
         default: break;
       }

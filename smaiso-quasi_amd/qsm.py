@@ -30,7 +30,8 @@ QUASI, STB = 0, 1
 LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "lapgrad_x",
             "lapgrad_y", "lapgrad_z", "sh", "rhostar", "divv",
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
-            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3", "stress3_divv"]
+            "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3", "stress3_divv",
+            "nonlin_py"]
 STORE_OPS = ["plain", "filter", "update", "update_stb"]
 
 FIELDS = [

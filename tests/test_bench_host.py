@@ -38,9 +38,9 @@ def test_kernel_symbols_match_rocprof_names():
     assert bench.kernel_symbol(dict(k, axis=0, ld=42), shape, False) == \
         "void qsm::k_group<double, 512, 2>(qsm::GroupArgs)"
     assert bench.kernel_symbol(dict(k, axis=2, inv=0, ld=26), shape, True) == \
-        "void qsm::k_multi<float, 512, 26, true>(qsm::MultiArgs)"
+        "void qsm::k_multi<float, 512, 26, 2>(qsm::MultiArgs)"
     assert bench.kernel_symbol(dict(k, axis=2, inv=0, ld=25), shape, True) == \
-        "void qsm::k_multi<float, 512, 25, false>(qsm::MultiArgs)"
+        "void qsm::k_multi<float, 512, 25, 0>(qsm::MultiArgs)"
     assert bench.kernel_symbol(dict(k, axis=0, inv=1, variant=4), shape, False) == \
         "void qsm::k_pass_pipe<double, 512, 0, true, 4>(qsm::PassArgs, int)"
 

@@ -550,9 +550,12 @@ def test_tile_variants_agree(qsm_mod, prec):
         # precision past the divergence switch the transverse velocity
         # components are round-off-sized remainders (v amplifies round-off
         # ~1e6 there), so v is held as one vector field
+        # (N-hat carries the pressure term p kp |grad rho|, whose p amplifies
+        # round-off likewise: measured 1.0e-5 between two mixed runs that
+        # round differently, against ~1e-7 for psi)
         if prec == 0 or f[0] != "V":
             b = tuned.field(f)
-            tol = (1e-13 if f[0] != "V" else 1e-8) if prec == 0 else 1e-5
+            tol = (1e-13 if f[0] != "V" else 1e-8) if prec == 0 else (1e-4 if f == "NHAT" else 1e-5)
             assert rel_l2(b, a) < tol, (f, rel_l2(b, a))
     if prec == 1:
         va = np.stack([base.field(f) for f in ("VX", "VY", "VZ")])
