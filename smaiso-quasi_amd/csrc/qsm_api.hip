@@ -28,6 +28,17 @@ namespace {
 
 thread_local std::string g_err;
 
+// n elements of esz bytes with pad elements of (unused) padding on either
+// side; returns the first element (null on failure)
+void* alloc_padded(size_t n, size_t esz, size_t pad) {
+  void* d = nullptr;
+  if (hipMalloc(&d, (n + 2 * pad) * esz) != hipSuccess) return nullptr;
+  return (char*)d + pad * esz;
+}
+void free_padded(void* p, size_t esz, size_t pad) {
+  if (p) hipFree((char*)p - pad * esz);
+}
+
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -360,6 +371,7 @@ struct qsm_handle_s {
   double* psihat = nullptr;   // FP64 state
   double* psihat_old = nullptr;
   float* psihat32 = nullptr;  // mixed precision: psihat rounded to FP32 (the spectral prologues' source)
+  size_t pad = 0;             // elements of padding before and after each field (alloc_padded)
   bool divv_valid = false;    // the D field holds IDCT(scale D-hat) of the current step
   bool p_valid = false;       // the p field holds the current step's pressure
   bool r_valid = false;       // the grad rho fields hold the current step's (lagged) gradient
@@ -954,10 +966,10 @@ int stb_growth(qsm_handle_s* h) {
 
 void free_handle(qsm_handle_s* h) {
   if (!h) return;
-  for (int f = 0; f < F_COUNT; ++f) if (h->f[f]) hipFree(h->f[f]);
-  if (h->psihat) hipFree(h->psihat);
-  if (h->psihat_old) hipFree(h->psihat_old);
-  if (h->psihat32) hipFree(h->psihat32);
+  for (int f = 0; f < F_COUNT; ++f) free_padded(h->f[f], h->esz, h->pad);
+  free_padded(h->psihat, 8, h->pad);
+  free_padded(h->psihat_old, 8, h->pad);
+  free_padded(h->psihat32, 4, h->pad);
   if (h->part) hipFree(h->part);
   if (h->dsc) hipFree(h->dsc);
   for (auto& g : h->graphs) hipGraphExecDestroy(g.second);
@@ -1069,11 +1081,20 @@ extern "C" int qsm_create(const qsm_params* pp, int prec, qsm_handle* out) {
     h->tw[a] = twiddles(nn[a], h->fp32);
     if (!h->tw[a]) return bail(QSM_ERR_NOMEM, "twiddle table");
   }
+  // Every field and psihat copy carries one y-z slab (ny nz elements) of
+  // padding on either side: the spectral operators' shifted-neighbour reads
+  // (+-1 along z, +-nz along y, +-ny nz along x, and sums of an x and a y or
+  // z shift, all within one slab) stay inside the allocation at the field's
+  // ends, so they read at a uniform offset with no address clamp and mask
+  // the values instead (qsm_ops.hpp).
+  h->pad = (size_t)p.ny * p.nz;
   for (int f = 0; f < F_COUNT; ++f)
-    if (hipMalloc(&h->f[f], h->n * h->esz) != hipSuccess) return bail(QSM_ERR_NOMEM, "field");
-  if (hipMalloc(&h->psihat, h->n * 8) != hipSuccess || hipMalloc(&h->psihat_old, h->n * 8) != hipSuccess)
+    if (!(h->f[f] = alloc_padded(h->n, h->esz, h->pad))) return bail(QSM_ERR_NOMEM, "field");
+  if (!(h->psihat = (double*)alloc_padded(h->n, 8, h->pad)) ||
+      !(h->psihat_old = (double*)alloc_padded(h->n, 8, h->pad)))
     return bail(QSM_ERR_NOMEM, "psihat");
-  if (h->fp32 && hipMalloc(&h->psihat32, h->n * 4) != hipSuccess) return bail(QSM_ERR_NOMEM, "psihat32");
+  if (h->fp32 && !(h->psihat32 = (float*)alloc_padded(h->n, 4, h->pad)))
+    return bail(QSM_ERR_NOMEM, "psihat32");
   long nb = pass_blocks(h->fp32, p.nz, 2, h->geo);
   h->part_n = nb > 4096 ? nb : 4096;
   if (hipMalloc(&h->part, h->part_n * 4 * 8) != hipSuccess) return bail(QSM_ERR_NOMEM, "partials");

@@ -116,9 +116,10 @@ __device__ __forceinline__ void st(void* base, Ix idx, T v) {
 
 // Split addressing: element thr + uni of a field, with thr a per-thread
 // offset (>= 0) and uni an offset uniform across the wavefront (may be
-// negative; thr + uni >= 0).  The uniform part is added to the base pointer
-// with scalar instructions and the per-thread part is the 32-bit vector
-// offset, so the accesses of a walk that share thr share one offset register.
+// negative; thr + uni >= 0, or within a padded field's padding).  The
+// uniform part is added to the base pointer with scalar instructions and the
+// per-thread part is the 32-bit vector offset, so the accesses of a walk that
+// share thr share one offset register.
 template <typename T>
 __device__ __forceinline__ T ld_at(const void* base, Ix thr, int uni) {
   const char* b = (const char*)base + (long)uni * (long)sizeof(T);
@@ -142,12 +143,13 @@ __device__ __forceinline__ double rd(const void* p, Ix idx) {
   return (double)ld<T>(p, idx);
 }
 
-// p[idx + off] if ok, else 0, without a branch: the load always reads a valid
-// address (idx itself when !ok) and the result is selected afterwards, so
-// the loads of one element can be issued together.
+// p[idx + off] if ok, else 0, without a branch: the load always reads
+// idx + off, inside the field or its padding (a step's fields carry one y-z
+// slab of padding on either side, and |off| <= ny nz), at a uniform offset,
+// and the value is selected afterwards.
 template <typename T>
 __device__ __forceinline__ double rd_if(const void* p, Ix idx, int off, bool ok) {
-  const double v = (double)ld<T>(p, ok ? idx + (Ix)off : idx);
+  const double v = (double)ld_at<T>(p, idx, off);
   return ok ? v : 0.0;
 }
 
@@ -206,9 +208,11 @@ __device__ __forceinline__ double grad_hat(const LoadOp& L, const Geo& G, const 
 // The projection prologues (pressure and the three velocity components) read
 // shifted neighbours of three or four fields and do FP64 arithmetic with one
 // division.  They run in two phases: fetch issues every load of an element
-// (raw values at valid addresses, nothing else), finish selects and
-// evaluates.  A pass fetches a chunk of elements before finishing any, so
-// the chunk's loads are in flight together.
+// (raw values at the shifted positions, inside the field or its padding: the
+// shifts stay within one y-z slab), finish selects and evaluates.  A pass
+// fetches a chunk of elements before finishing any, so the chunk's loads are
+// in flight together.  The offsets are uniform (the P-rank halo slab of
+// v_y and v_z aside), so a load costs no vector address arithmetic.
 template <int LD>
 constexpr bool load_split() {
   return LD == LD_PRESSURE || LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2;
@@ -240,21 +244,25 @@ __device__ __forceinline__ Shifts shifts(const LoadOp& L, const Geo& G, int i, i
     const bool in = i < G.nx - 1;
     return Shifts{true, band(in, j > 0), band(in, k > 0), band(in, hasd), 0, nyz - G.nz, nyz - 1, nyz};
   } else {
-    // zero at j = Ny-1 or k = Nz-1.  The S_x neighbour slab is i-1, or the
-    // P-rank halo slab (quasi.cpp:1639-1650).
+    // zero at j = Ny-1 or k = Nz-1.  The S_x neighbour slab is i-1 here
+    // (exact stencil); fetch reads the P-rank halo slab instead where that
+    // map applies (halo_slab).
     const bool in = (j < G.ny - 1) & (k < G.nz - 1);
-    int dx = -nyz;  // exact stencil: slab i-1
-    if (L.n0 > 0) {  // uniform: the P-rank halo map
-      // i mod n0 without an integer division: floor((i + 1/2)/n0) in FP32 is
-      // exact for i < 2^22 (relative error 2^-23 against a margin of 1/(2 n0))
-      const int qn = (int)(((float)i + 0.5f) * (1.0f / (float)L.n0));
-      if ((i > 0) & (i == qn * L.n0)) dx = -L.n0 * nyz;
-    }
     if constexpr (LD == LD_VEL1)  // a: S_y, b: S_x[ih][j+1], c: S_z[j+1][k-1], d: D[j+1]
-      return Shifts{true, band(in, i > 0), band(in, k > 0), band(in, hasd), 0, dx + G.nz, G.nz - 1, G.nz};
+      return Shifts{true, band(in, i > 0), band(in, k > 0), band(in, hasd), 0, G.nz - nyz, G.nz - 1, G.nz};
     else  // a: S_z, b: S_x[ih][k+1], c: S_y[j-1][k+1], d: D[k+1]
-      return Shifts{true, band(in, i > 0), band(in, j > 0), band(in, hasd), 0, dx + 1, 1 - G.nz, 1};
+      return Shifts{true, band(in, i > 0), band(in, j > 0), band(in, hasd), 0, 1 - nyz, 1 - G.nz, 1};
   }
+}
+
+// The P-rank halo map of v_y and v_z (quasi.cpp:1639-1650): at the first slab
+// of a rank's slab range (i mod n0 = 0, i > 0) the S_x neighbour is slab
+// i - n0 instead of i - 1.  i mod n0 without an integer division:
+// floor((i + 1/2)/n0) in FP32 is exact for i < 2^22 (relative error 2^-23
+// against a margin of 1/(2 n0)).
+__device__ __forceinline__ bool halo_slab(const LoadOp& L, int i) {
+  const int qn = (int)(((float)i + 0.5f) * (1.0f / (float)L.n0));
+  return (i > 0) & (i == qn * L.n0);
 }
 
 template <typename T, int LD>
@@ -267,11 +275,16 @@ __device__ __forceinline__ Raw4<T> fetch(const LoadOp& L, const Geo& G, int i, i
   const void* pc = LD == LD_VEL2 ? L.a[1] : L.a[2];
   const void* pd = L.a[3] ? L.a[3] : L.a[0];  // a valid address when D-hat is null
   Raw4<T> r;
-  r.a = ld<T>(pa, h.fa ? idx + (Ix)h.oa : idx);
-  r.b = ld<T>(pb, h.fb ? idx + (Ix)h.ob : idx);
-  r.c = ld<T>(pc, h.fc ? idx + (Ix)h.oc : idx);
-  if constexpr (LD != LD_PRESSURE) r.d = ld<T>(pd, h.fd ? idx + (Ix)h.od : idx);
+  r.a = ld_at<T>(pa, idx, h.oa);
+  r.b = ld_at<T>(pb, idx, h.ob);
+  r.c = ld_at<T>(pc, idx, h.oc);
+  if constexpr (LD != LD_PRESSURE) r.d = ld_at<T>(pd, idx, h.od);
   else r.d = (T)0;
+  if constexpr (LD == LD_VEL1 || LD == LD_VEL2) {
+    if (L.n0 > 0) {  // uniform: the P-rank halo map
+      if (halo_slab(L, i)) r.b = ld_at<T>(pb, idx, h.ob - (L.n0 - 1) * G.ny * G.nz);
+    }
+  }
   return r;
 }
 
@@ -465,7 +478,7 @@ __device__ __forceinline__ T load_eval(const LoadOp& L, const Phys& P, const Kv&
     const int nc = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
     const int o = c == 0 ? G.ny * G.nz : (c == 1 ? G.nz : 1);
     const bool ok = m < nc - 1;
-    const T x = ld<T>(L.a[0], ok ? idx + (Ix)o : idx);
+    const T x = ld_at<T>(L.a[0], idx, o);  // inside the field or its padding
     return ok ? (T)f * x : (T)0;
   } else if constexpr (LD == LD_COEF_F || LD == LD_COEF_G) {
     const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
