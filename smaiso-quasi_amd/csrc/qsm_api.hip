@@ -378,7 +378,7 @@ struct qsm_handle_s {
   bool l_yspec = false;       // the lap grad psi fields hold their y-passed values (the stress
                               // pass completed their inverse z passes in registers); the
                               // real-space fields are formed on request (lapgrad_output)
-  bool r_yspec = false;       // the same for grad rho (stress and nonlinear passes)
+  bool r_yspec = false;       // the same for grad rho (stress pass)
   bool p_yspec = false;       // the same for p (nonlinear pass)
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
@@ -429,7 +429,8 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
     case LD_VEL0: case LD_VEL1: case LD_VEL2: rd = (L.a[3] ? 4 : 3) * e; break;
     case LD_SCALE: rd = e; break;
     case LD_NONLIN: rd = 10 * e; break;
-    case LD_NONLIN_P: case LD_NONLIN_PY: rd = 14 * e; break;
+    case LD_NONLIN_P: rd = 14 * e; break;
+    case LD_NONLIN_PY: rd = 12 * e; break;
     case LD_STRESS_STB: rd = 3 * e; break;
     case LD_NONLIN_STB: rd = 7 * e; break;
     case LD_NONLIN_STB0: rd = e; break;
@@ -591,8 +592,9 @@ int inv3_group(qsm_handle_s* h, int group, const GroupItem* it, int count, int r
 // inputs once (launch_multi), then y and x in place per value.  `reads`:
 // distinct fields the operator reads (algorithmic bytes).  Returns 1 (no
 // error set) where the pencil length has no multi-output kernel.
+// aux (lz = 2): the pass also writes |grad rho|^2 of each element there.
 int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int reads,
-               int lz = 0) {
+               int lz = 0, void* aux = nullptr) {
   if (!multi_supported(h->fp32, h->p.nz)) return 1;
   if (!h->dry) {
     if (h->rec) return fail(QSM_ERR_STATE, "multi-output passes cannot be recorded");
@@ -603,6 +605,7 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int
     A.tw = h->tw[2];
     A.dmask = 1 << 2;  // component z: DST along z
     A.lz = lz;
+    A.aux = lz == 2 ? aux : nullptr;
     for (int c = 0; c < nout; ++c) A.out[c] = out[c];
     Timer& T = h->timer;
     const int key = timer_key(2, false, L.code, ST_PLAIN, 0);
@@ -613,7 +616,8 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int
     if (e != hipSuccess) return fail(QSM_ERR_HIP, "multi-output launch: %s", hipGetErrorString(e));
     if (timed) {
       hipEventRecord(eb, h->stream);
-      T.recs.push_back({ea, eb, key, (double)(reads + nout) * (double)h->n * (double)h->esz});
+      const int writes = nout + (A.aux ? 1 : 0);
+      T.recs.push_back({ea, eb, key, (double)(reads + writes) * (double)h->n * (double)h->esz});
     }
   }
   for (int c = 0; c < nout; ++c) {
@@ -664,8 +668,9 @@ int quasi_step(qsm_handle_s* h) {
   const bool grp = !h->rec;  // group kernels (single handles)
   // Single handles form the stress components (and D) in one multi-output z
   // pass.  That pass completes the inverse transforms of lap grad psi and,
-  // once D is live, of grad rho; the nonlinear pass completes those of grad
-  // rho and p.  Their z passes are not run separately.
+  // once D is live, of grad rho, whose |grad rho|^2 it leaves in the D field
+  // for the nonlinear pass; that pass completes the transform of p.  Their z
+  // passes are not run separately.
   const bool multi = grp && multi_supported(h->fp32, p.nz);
   h->divv_valid = false;
   h->p_valid = plive && !multi;
@@ -748,7 +753,11 @@ int quasi_step(qsm_handle_s* h) {
     L.a[9] = rho_old;
     for (int c = 0; c < 3; ++c) { L.a[10 + c] = F(h, F_VX + c); L.a[13 + c] = F(h, F_RX + c); }
     void* const out[4] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ), F(h, F_DHAT)};
-    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9, dlive ? 2 : 1))) return rc;
+    // with D live the pass also forms |grad rho|^2 for the nonlinear pass's
+    // pressure term, in the D field (read by no stage; formed on request)
+    if ((rc = fwd3_multi(h, L, out, dlive ? 4 : 3, dlive ? 16 : 9, dlive ? 2 : 1,
+                         dlive ? F(h, F_DIVV) : nullptr)))
+      return rc;
   } else {
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS; L.axis = c;
@@ -793,6 +802,7 @@ int quasi_step(qsm_handle_s* h) {
   L.a[4] = F(h, F_VX); L.a[5] = F(h, F_VY); L.a[6] = F(h, F_VZ);
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
   L.a[10] = F(h, F_P); L.a[11] = F(h, F_RX); L.a[12] = F(h, F_RY); L.a[13] = F(h, F_RZ);
+  if (L.code == LD_NONLIN_PY) L.a[11] = F(h, F_DIVV);  // |grad rho|^2 from the stress pass
   S = StoreOp{}; S.code = ST_UPDATE; S.first = n == 1;
   S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.d[2] = h->psihat32; S.s[0] = h->psihat;
   if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }

@@ -103,6 +103,7 @@ struct MultiArgs {
   int dmask;
   int lz;          // 1: a[6..8] (lap grad psi) are y-passed, 2: a[13..15] (grad rho) too;
                    // the pass runs their inverse z rounds
+  void* aux;       // lz = 2: |grad rho|^2 of each element (null: not formed)
 };
 hipError_t launch_multi(int prec_fp32, int n, const MultiArgs& A, long* nblocks, hipStream_t s);
 bool multi_supported(int prec_fp32, int n);

@@ -66,8 +66,9 @@ enum LoadCode : int {
   LD_STRESS_STB3,   // LD_STRESS_STB for c = x, y, z; a: sh, psi, g_x,y,z
   LD_STRESS3_DIVV,  // LD_STRESS3, then LD_DIVV as a fourth value; a: as LD_STRESS3, then
                     //   rho_old, v_x,y,z, r_x,y,z (rho is a0)
-  LD_NONLIN_PY,     // LD_NONLIN_P with p (a10) and grad rho (a11-13) y-passed: the pass
-                    //   completes their inverse z transforms and keeps the pressure term
+  LD_NONLIN_PY,     // LD_NONLIN_P with p (a10) y-passed and |grad rho|^2 (a11) formed by
+                    //   the stress pass: the pass completes p's inverse z transform and
+                    //   keeps the pressure term
 };
 
 enum StoreCode : int {

@@ -481,50 +481,45 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   double dsum = 0.0;
   cx<T> v[E];
 
-  // LD_NONLIN_PY: p (a10) and grad rho (a11-13) arrive y-passed.  Four
-  // inverse z rounds over the tile (r_x, r_y, r_z with DST-III along z,
-  // then p) complete their transforms; each thread folds the values of its
-  // walk elements into the pressure term p kp |grad rho| / rho_s^2, the one
-  // value the operator then takes from registers.  Their separate inverse z
-  // passes are not run.
+  // LD_NONLIN_PY: p (a10) arrives y-passed, and |grad rho|^2 (a11) as the
+  // stress pass formed it from the grad rho values it transformed in
+  // registers.  One inverse z round over the tile completes p's transform;
+  // each thread folds the values of its walk elements into the pressure term
+  // p kp |grad rho| / rho_s^2, the one value the operator then takes from
+  // registers.  p's separate inverse z pass is not run.
   T pterm[pre_rounds(LD) ? TK * N / BLOCK : 1];
   if constexpr (pre_rounds(LD)) {
     constexpr int PER = TK * N / BLOCK;
     WalkPos<AX, N, TK, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
-    WalkPos<AX, N, TK, TKP, BLOCK, 2> wr;  // reversed (DST-III)
     WalkPos<AX, N, TK, TKP, BLOCK, 1> wq;  // the transform's output positions
     wn.init(tid);
-    wr.init(tid);
     wq.init(tid);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const void* src = Lop.a[c < 3 ? 11 + c : 10];
-#pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        int q, n, i, j, k;
-        Ix idx;
-        bool ok, oddn;
-        tg.walk(tid, s, q, n, i, j, k, idx, ok);
-        T x;
-        if constexpr (SPLIT) x = ld_at<T>(src, tg.wbase, walk_uni(s));
-        else x = ld<T>(src, idx);
-        rbuf[c == 2 ? wr.pos(tid, s, oddn) : wn.pos(tid, s, oddn)] = x;
+    for (int s = 0; s < PER; ++s) {
+      int q, n, i, j, k;
+      Ix idx;
+      bool ok, oddn;
+      tg.walk(tid, s, q, n, i, j, k, idx, ok);
+      T x;
+      if constexpr (SPLIT) {
+        x = ld_at<T>(Lop.a[10], tg.wbase, walk_uni(s));
+        pterm[s] = ld_at<T>(Lop.a[11], tg.wbase, walk_uni(s));
+      } else {
+        x = ld<T>(Lop.a[10], idx);
+        pterm[s] = ld<T>(Lop.a[11], idx);
       }
-      __syncthreads();
-      cx<T> vr[E];
-      inv_preprocess<T, N, M, E, TKP, R1>(vr, rbuf, tp, p, t);
-      run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, vr, buf, p, t, ts);
-#pragma unroll
-      for (int s = 0; s < PER; ++s) {
-        bool oddn;
-        T y = rbuf[wq.pos(tid, s, oddn)];
-        if (c == 2 && oddn) y = -y;
-        if (c == 0) pterm[s] = y * y;
-        else if (c < 3) pterm[s] = pterm[s] + y * y;
-        else pterm[s] = pressure_term<T>(y, pterm[s], Ph);
-      }
-      __syncthreads();  // the next round (or the load phase) rewrites the tile
+      rbuf[wn.pos(tid, s, oddn)] = x;
     }
+    __syncthreads();
+    cx<T> vr[E];
+    inv_preprocess<T, N, M, E, TKP, R1>(vr, rbuf, tp, p, t);
+    run_fft<T, N, M, E, TKP, 1, true>(typename PL::R{}, vr, buf, p, t, ts);
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      bool oddn;
+      pterm[s] = pressure_term<T>(rbuf[wq.pos(tid, s, oddn)], pterm[s], Ph);
+    }
+    __syncthreads();  // the load phase rewrites the tile
     // the forward post-processing twiddles, read after the load phase's barrier
     const cx<T>* __restrict__ tw = (const cx<T>*)twp;
     for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
@@ -992,7 +987,21 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
     using IR = MultiRound<T, N, R, TKP, BLOCK, M, E, R1, typename PL::R>;
     IR::template run<6, 7, 8, (LZ == 2 ? 13 : -1), 2, 0>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t,
                                                         tid, lz);
-    if constexpr (LZ == 2) IR::template run<14, 15, -1, -1, 1, 4>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t, tid, lz);
+    if constexpr (LZ == 2) {
+      IR::template run<14, 15, -1, -1, 1, 4>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t, tid, lz);
+      // |grad rho|^2 for the nonlinear pass's pressure term (the expression
+      // of LD_NONLIN_P), so that pass need not transform grad rho again
+      if (A.aux) {
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+          if (!row_ok(s)) continue;
+          const T rx = lz[s][3], ry = lz[s][4], rz = lz[s][5];
+          const T r2 = rx * rx + ry * ry + rz * rz;
+          if constexpr (SPLIT) st_at<T>(A.aux, tg.wbase, s * BLOCK, r2);
+          else st<T>(A.aux, walk_idx(s), r2);
+        }
+      }
+    }
     for (int u = tid; u < TW::NPOST; u += BLOCK) tp[u] = tw[TW::POST + u];
   }
   {
