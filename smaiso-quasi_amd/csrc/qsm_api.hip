@@ -328,8 +328,12 @@ namespace {
 enum Fld {
   F_NHAT_OLD, F_NTMP, F_RHOHAT, F_RHO_A, F_RHO_B, F_VX, F_VY, F_VZ, F_PSI_A, F_PSI_B,
   F_F, F_G, F_GX, F_GY, F_GZ, F_RX, F_RY, F_RZ, F_LX, F_LY, F_LZ, F_DHAT, F_DIVV,
-  F_SX, F_SY, F_SZ, F_P, F_COUNT
+  F_SX, F_SY, F_SZ, F_P,
+  // psi's x-passed and y-passed fields (quasi): the transforms of grad psi y
+  // and z start from them.  Derived from psihat; not in checkpoints.
+  F_PSIX, F_PSIY, F_COUNT
 };
+constexpr int F_CKPT = F_PSIX;  // fields a checkpoint holds: those before F_PSIX
 
 // device FP64 scalars
 enum Dsc {
@@ -636,6 +640,36 @@ const void* psihat_src(const qsm_handle_s* h) {
   return h->fp32 ? (const void*)h->psihat32 : (const void*)h->psihat;
 }
 
+// psi's x- and y-passed fields from psihat (quasi): the first two passes of
+// psi = IDCT(psihat), kept for the transforms of grad psi y and z.
+int psi_xy(qsm_handle_s* h) {
+  int rc = run_pass(h, 0, true, false, plain_load(psihat_src(h)), plain_store(F(h, F_PSIX)), true, nullptr);
+  if (!rc) rc = run_pass(h, 1, true, false, plain_load(F(h, F_PSIX)), plain_store(F(h, F_PSIY)), false, nullptr);
+  return rc;
+}
+
+// psi = IDCT(psihat) into out (quasi.cpp:924-927, 1961); the quasi variant
+// passes through F_PSIX and F_PSIY, which the next step reads.
+int psi_inverse(qsm_handle_s* h, void* out) {
+  if (h->p.variant != QSM_QUASI) return inv3(h, -1, plain_load(psihat_src(h)), out);
+  const int rc = psi_xy(h);
+  return rc ? rc : run_pass(h, 2, true, false, plain_load(F(h, F_PSIY)), plain_store(out), false, nullptr);
+}
+
+// grad psi y and z (quasi.cpp:1060-1118) from psi's x- and y-passed fields:
+// -s_c psihat[m+1] (0 at m = N_c - 1) commutes with the transforms along the
+// other axes, so grad psi y is the y pass of the gradient operator on
+// F_PSIX, then the z pass; grad psi z the z pass of the operator on F_PSIY.
+int grad_psi_yz(qsm_handle_s* h) {
+  LoadOp L{};
+  L.code = LD_GRAD_Y; L.axis = 1; L.a[0] = F(h, F_PSIX);
+  int rc = run_pass(h, 1, true, true, L, plain_store(F(h, F_GY)), true, nullptr);
+  if (!rc) rc = run_pass(h, 2, true, false, plain_load(F(h, F_GY)), plain_store(F(h, F_GY)), false, nullptr);
+  if (rc) return rc;
+  L.code = LD_GRAD_Z; L.axis = 2; L.a[0] = F(h, F_PSIY);
+  return run_pass(h, 2, true, true, L, plain_store(F(h, F_GZ)), true, nullptr);
+}
+
 // Whether the grad rho arrays hold the gradient that step n reads: steps
 // after divv_switch form it; at step 1 it is that of rhohat = 0, the zeros
 // the arrays hold from set-up; other steps do not form it (quasi_step).
@@ -680,21 +714,23 @@ int quasi_step(qsm_handle_s* h) {
   h->l_yspec = multi;
   LoadOp L{};
   if (grp) {
-    // S1 + S2 (quasi.cpp:1034-1232): F, G, grad psi, laplacian of grad psi
-    // from psihat in one group; grad rho (lagged) from rhohat in another
-    GroupItem it[8];
+    // S1 + S2 (quasi.cpp:1034-1232): F, G, grad psi x, laplacian of grad
+    // psi from psihat in one group, grad psi y and z from psi's passed
+    // fields; grad rho (lagged) from rhohat in another group
+    GroupItem it[6];
     it[0] = {LoadOp{}, F(h, F_F), -1};
     it[0].L.code = LD_COEF_F;
     it[1] = {LoadOp{}, F(h, F_G), -1};
     it[1].L.code = LD_COEF_G;
+    it[2] = {LoadOp{}, F(h, F_GX), 0};
+    it[2].L.code = LD_GRAD_X;
     for (int c = 0; c < 3; ++c) {
-      it[2 + c] = {LoadOp{}, F(h, F_GX + c), c};
-      it[2 + c].L.code = LD_GRAD_X + c;
-      it[5 + c] = {LoadOp{}, F(h, F_LX + c), c};
-      it[5 + c].L.code = LD_LAPGRAD_X + c;
+      it[3 + c] = {LoadOp{}, F(h, F_LX + c), c};
+      it[3 + c].L.code = LD_LAPGRAD_X + c;
     }
     for (auto& x : it) { x.L.a[0] = psih; x.L.axis = x.dst_axis; }
-    if ((rc = inv3_group(h, GRP_PSI, it, 8, 1, multi ? 0xE0u : 0u))) return rc;
+    if ((rc = inv3_group(h, GRP_PSI, it, 6, 1, multi ? 0x38u : 0u))) return rc;
+    if ((rc = grad_psi_yz(h))) return rc;
     if (dlive) {
       for (int c = 0; c < 3; ++c) {
         it[c] = {LoadOp{}, F(h, F_RX + c), c};
@@ -712,9 +748,12 @@ int quasi_step(qsm_handle_s* h) {
     if ((rc = inv3(h, -1, L, F(h, F_G)))) return rc;
 
     // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
+    if ((rc = grad_psi_yz(h))) return rc;
     for (int c = 0; c < 3; ++c) {
-      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
-      if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
+      if (c == 0) {
+        L = LoadOp{}; L.code = LD_GRAD_X; L.axis = 0; L.a[0] = psih;
+        if ((rc = inv3(h, 0, L, F(h, F_GX)))) return rc;
+      }
       if (dlive) {
         L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
         if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
@@ -810,7 +849,7 @@ int quasi_step(qsm_handle_s* h) {
   std::swap(h->psihat, h->psihat_old);
 
   // psi = IDCT(psihat) into the psi_old buffer, then swap (psi_old <- psi)
-  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_old)))) return rc;
+  if ((rc = psi_inverse(h, F(h, h->psi_old)))) return rc;
   std::swap(h->psi_cur, h->psi_old);
   return QSM_OK;
 }
@@ -1142,7 +1181,7 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   HIPCHK(hipMemcpyAsync(h->psihat_old, h->psihat, n * 8, hipMemcpyDeviceToDevice, h->stream));
   if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)n, h->stream));
   // psi = IDCT(psihat) (quasi.cpp:924-927), psi_old = psi
-  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_cur)))) return rc;
+  if ((rc = psi_inverse(h, F(h, h->psi_cur)))) return rc;
   HIPCHK(hipMemcpyAsync(F(h, h->psi_old), F(h, h->psi_cur), n * h->esz, hipMemcpyDeviceToDevice, h->stream));
   if (h->p.variant == QSM_STB) {
     // rho = 1 in the pure smectic (quasi-stb.cpp:1232): kept for the mass diagnostic
@@ -1922,7 +1961,7 @@ extern "C" int qsm_checkpoint_write(qsm_handle h, const char* path) {
   put(h->dsc, D_COUNT * 8);
   const int order[] = {h->psi_cur, h->psi_old, h->rho_cur, h->rho_old};
   for (int f : order) put(h->f[f], h->n * h->esz);
-  for (int f = 0; f < F_COUNT; ++f)
+  for (int f = 0; f < F_CKPT; ++f)
     if (f != F_PSI_A && f != F_PSI_B && f != F_RHO_A && f != F_RHO_B) put(h->f[f], h->n * h->esz);
   ok = (fclose(fp) == 0) && ok;
   return ok ? QSM_OK : fail(QSM_ERR_HIP, "qsm_checkpoint_write: write failed (%s)", path);
@@ -1951,11 +1990,15 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   get(h->dsc, D_COUNT * 8);
   const int order[] = {h->psi_cur, h->psi_old, h->rho_cur, h->rho_old};
   for (int f : order) get(h->f[f], h->n * h->esz);
-  for (int f = 0; f < F_COUNT; ++f)
+  for (int f = 0; f < F_CKPT; ++f)
     if (f != F_PSI_A && f != F_PSI_B && f != F_RHO_A && f != F_RHO_B) get(h->f[f], h->n * h->esz);
   fclose(fp);
   if (!ok) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is truncated", path);
   if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)h->n, h->stream));
+  if (h->p.variant == QSM_QUASI) {
+    const int rc = psi_xy(h);  // psi's passed fields, by the passes the step runs
+    if (rc) return rc;
+  }
   h->step = hd.step;
   h->divv_valid = false;
   h->p_valid = false;

@@ -67,7 +67,7 @@ long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 // another on its tile, so later outputs re-read the shared inputs from the
 // caches.
 enum GroupCode : int {
-  GRP_PSI = 0,     // F, G, grad psi (x, y, z), lap grad psi (x, y, z) from psihat
+  GRP_PSI = 0,     // F, G, grad psi x, lap grad psi (x, y, z) from psihat
   GRP_RHO,         // grad rho (x, y, z) from rhohat
   GRP_PROJ,        // p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
   GRP_STB_PSI,     // sh, grad psi (x, y, z)

@@ -1234,12 +1234,13 @@ template <int G>
 struct GroupOps;
 template <>
 struct GroupOps<GRP_PSI> {
-  using L = OpList<LD_COEF_F, LD_COEF_G, LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z, LD_LAPGRAD_X,
-                   LD_LAPGRAD_Y, LD_LAPGRAD_Z>;
+  // grad psi y and z are not here: their transforms start from psi's x- and
+  // y-passed fields (the y or z pass with the gradient operator)
+  using L = OpList<LD_COEF_F, LD_COEF_G, LD_GRAD_X, LD_LAPGRAD_X, LD_LAPGRAD_Y, LD_LAPGRAD_Z>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
-  static constexpr int PARTS = 2;  // 1495 -> 1420 us (4 and 8 parts: slower)
+  static constexpr int PARTS = 2;
   static constexpr int VAR64 = 1;
   static constexpr bool FUSED = false;
 };
@@ -1622,6 +1623,13 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
         default: break;
       }
     }
+  }
+  if constexpr (STEP && AX != kAxX) {
+    // the gradient along the pass axis of a field whose earlier axes are
+    // already transformed (grad psi y from psi's x-passed field, z from its
+    // y-passed one)
+    constexpr int GLD = AX == kAxY ? LD_GRAD_Y : LD_GRAD_Z;
+    if (inv && st == ST_PLAIN && ld == GLD) return launch_one<T, N, AX, true, GLD, ST_PLAIN>(A, nb, s, B);
   }
   if constexpr (STEP && AX == kAxX) {
     if (!inv && ld == LD_PLAIN) {

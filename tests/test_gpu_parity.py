@@ -270,12 +270,16 @@ def test_fields_formed_on_request(qsm_mod, oracle_mod, prec):
             # FP64 per component.  Mixed: the vector as a whole, since psi
             # here varies in x and y only through its 1e-4 noise, and the x
             # and y components of grad rho (~1e-4 of rho's range) are at the
-            # resolution of FP32 rho (~1)
+            # resolution of FP32 rho (~1).  Those two components are 1e-5 of
+            # the vector's norm: in the oracle itself, psi perturbed by one
+            # FP64 rounding (1e-16 relative) moves them by up to 4e-11 at
+            # this step, so their FP64 bound is 1e-9 (the z component 1e-10)
             got = np.stack([sim.field(f) for f in ("RX", "RY", "RZ")])
             ref = np.stack([orc.field(f) for f in ("RX", "RY", "RZ")])
             errs = [rel_l2(got, ref)] if prec else [rel_l2(a, b) for a, b in zip(got, ref)]
+            tols = [tol] if prec else [1e-9, 1e-9, tol]
             print(f"prec {prec} step {n} grad rho: rel-L2 {errs}", flush=True)
-            assert max(errs) < tol, (n, errs)
+            assert all(e < t for e, t in zip(errs, tols)), (n, errs)
     for f in ("PSI", "PSIHAT", "RHO", "VX", "VY", "VZ", "NHAT", "P"):
         assert np.array_equal(sim.field(f), quiet.field(f)), f
     sim.close()
