@@ -330,8 +330,11 @@ enum Fld {
   F_F, F_G, F_GX, F_GY, F_GZ, F_RX, F_RY, F_RZ, F_LX, F_LY, F_LZ, F_DHAT, F_DIVV,
   F_SX, F_SY, F_SZ, F_P,
   // psi's x-passed and y-passed fields (quasi): the transforms of grad psi y
-  // and z start from them.  Derived from psihat; not in checkpoints.
-  F_PSIX, F_PSIY, F_COUNT
+  // and z start from them.  rho's x-passed field and its y-passed fields
+  // (ping-pong: the latest rho's and the one before, which the lagged grad
+  // rho reads): the same for grad rho y and z.  Derived from psihat and
+  // rhohat; not in checkpoints.
+  F_PSIX, F_PSIY, F_RHOX, F_RHOY0, F_RHOY1, F_COUNT
 };
 constexpr int F_CKPT = F_PSIX;  // fields a checkpoint holds: those before F_PSIX
 
@@ -384,6 +387,7 @@ struct qsm_handle_s {
                               // real-space fields are formed on request (lapgrad_output)
   bool r_yspec = false;       // the same for grad rho (stress pass)
   bool p_yspec = false;       // the same for p (nonlinear pass)
+  int rhoy = 0;               // F_RHOY0 + rhoy: y-passed field of the latest rho
   double* part = nullptr;     // per-block partial sums
   long part_n = 0;
   double* dsc = nullptr;
@@ -610,6 +614,7 @@ int fwd3_multi(qsm_handle_s* h, const LoadOp& L, void* const* out, int nout, int
     A.dmask = 1 << 2;  // component z: DST along z
     A.lz = lz;
     A.aux = lz == 2 ? aux : nullptr;
+    A.sz = h->waves.sz;
     for (int c = 0; c < nout; ++c) A.out[c] = out[c];
     Timer& T = h->timer;
     const int key = timer_key(2, false, L.code, ST_PLAIN, 0);
@@ -668,6 +673,41 @@ int grad_psi_yz(qsm_handle_s* h) {
   if (rc) return rc;
   L.code = LD_GRAD_Z; L.axis = 2; L.a[0] = F(h, F_PSIY);
   return run_pass(h, 2, true, true, L, plain_store(F(h, F_GZ)), true, nullptr);
+}
+
+// rho = IDCT(rhohat) into out (quasi.cpp:1263-1268) through F_RHOX and the
+// free one of the two y-passed fields, which then holds the latest rho's.
+int rho_inverse(qsm_handle_s* h, void* out) {
+  const int cur = h->rhoy ^ 1;
+  void* y = F(h, F_RHOY0 + cur);
+  int rc = run_pass(h, 0, true, false, plain_load(F(h, F_RHOHAT)), plain_store(F(h, F_RHOX)), true, nullptr);
+  if (!rc) rc = run_pass(h, 1, true, false, plain_load(F(h, F_RHOX)), plain_store(y), false, nullptr);
+  if (!rc && out) rc = run_pass(h, 2, true, false, plain_load(y), plain_store(out), false, nullptr);
+  if (!rc) h->rhoy = cur;
+  return rc;
+}
+
+// The lagged grad rho (quasi.cpp:1062-1118: that of the rhohat the step
+// starts from), before the step's density pass.  r_x from rhohat; r_y, r_z
+// from rho's x- and y-passed fields, which the previous step's rho_inverse
+// left (the gradient operator commutes with the transforms along the other
+// axes).  yspec: r_x and r_y stop after their y passes and r_z is not
+// formed; the stress pass completes them (r_z from F_RHOY0 + rhoy).
+int grad_rho(qsm_handle_s* h, bool yspec) {
+  LoadOp L{};
+  L.code = LD_GRAD_X; L.axis = 0; L.a[0] = F(h, F_RHOHAT);
+  void* rx = F(h, F_RX);
+  int rc = run_pass(h, 0, true, true, L, plain_store(rx), true, nullptr);
+  if (!rc) rc = run_pass(h, 1, true, false, plain_load(rx), plain_store(rx), false, nullptr);
+  if (!rc && !yspec) rc = run_pass(h, 2, true, false, plain_load(rx), plain_store(rx), false, nullptr);
+  if (rc) return rc;
+  L.code = LD_GRAD_Y; L.axis = 1; L.a[0] = F(h, F_RHOX);
+  void* ry = F(h, F_RY);
+  rc = run_pass(h, 1, true, true, L, plain_store(ry), true, nullptr);
+  if (!rc && !yspec) rc = run_pass(h, 2, true, false, plain_load(ry), plain_store(ry), false, nullptr);
+  if (rc || yspec) return rc;
+  L.code = LD_GRAD_Z; L.axis = 2; L.a[0] = F(h, F_RHOY0 + h->rhoy);
+  return run_pass(h, 2, true, true, L, plain_store(F(h, F_RZ)), true, nullptr);
 }
 
 // Whether the grad rho arrays hold the gradient that step n reads: steps
@@ -731,15 +771,7 @@ int quasi_step(qsm_handle_s* h) {
     for (auto& x : it) { x.L.a[0] = psih; x.L.axis = x.dst_axis; }
     if ((rc = inv3_group(h, GRP_PSI, it, 6, 1, multi ? 0x38u : 0u))) return rc;
     if ((rc = grad_psi_yz(h))) return rc;
-    if (dlive) {
-      for (int c = 0; c < 3; ++c) {
-        it[c] = {LoadOp{}, F(h, F_RX + c), c};
-        it[c].L.code = LD_GRAD_X + c;
-        it[c].L.axis = c;
-        it[c].L.a[0] = F(h, F_RHOHAT);
-      }
-      if ((rc = inv3_group(h, GRP_RHO, it, 3, 1, multi ? 0x7u : 0u))) return rc;
-    }
+    if (dlive && (rc = grad_rho(h, multi))) return rc;
   } else {
     // S1: F, G (quasi.cpp:1034-1057)
     L.code = LD_COEF_F; L.a[0] = psih;
@@ -749,15 +781,10 @@ int quasi_step(qsm_handle_s* h) {
 
     // S2: grad psi, grad rho (lagged), laplacian of grad psi (quasi.cpp:1060-1232)
     if ((rc = grad_psi_yz(h))) return rc;
+    L = LoadOp{}; L.code = LD_GRAD_X; L.axis = 0; L.a[0] = psih;
+    if ((rc = inv3(h, 0, L, F(h, F_GX)))) return rc;
+    if (dlive && (rc = grad_rho(h, false))) return rc;
     for (int c = 0; c < 3; ++c) {
-      if (c == 0) {
-        L = LoadOp{}; L.code = LD_GRAD_X; L.axis = 0; L.a[0] = psih;
-        if ((rc = inv3(h, 0, L, F(h, F_GX)))) return rc;
-      }
-      if (dlive) {
-        L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = F(h, F_RHOHAT);
-        if ((rc = inv3(h, c, L, F(h, F_RX + c)))) return rc;
-      }
       L = LoadOp{}; L.code = LD_LAPGRAD_X + c; L.axis = c; L.a[0] = psih;
       if ((rc = inv3(h, c, L, F(h, F_LX + c)))) return rc;
     }
@@ -769,7 +796,7 @@ int quasi_step(qsm_handle_s* h) {
   StoreOp S{}; S.code = ST_FILTER; S.d[0] = F(h, F_RHOHAT);
   if (mixed) { S.dc = h->dsc + D_RHOSUM; S.dc_out = h->dsc + D_RHOHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_RHOHAT), S, mixed ? D_RHOSUM : -1))) return rc;
-  if ((rc = inv3(h, -1, plain_load(F(h, F_RHOHAT)), rho))) return rc;
+  if ((rc = rho_inverse(h, rho))) return rc;
 
   // S4: divergence (quasi.cpp:1273-1293).  Up to divv_switch the reference
   // transforms an all-zero D; D-hat = 0 then, which the D-hat array already
@@ -791,6 +818,10 @@ int quasi_step(qsm_handle_s* h) {
     for (int c = 0; c < 3; ++c) { L.a[3 + c] = F(h, F_GX + c); L.a[6 + c] = F(h, F_LX + c); }
     L.a[9] = rho_old;
     for (int c = 0; c < 3; ++c) { L.a[10 + c] = F(h, F_VX + c); L.a[13 + c] = F(h, F_RX + c); }
+    // with D live the pass forms r_z from rho's y-passed field of the step
+    // before (the lagged gradient); the density pass has since written the
+    // other one
+    if (dlive) L.a[15] = F(h, F_RHOY0 + (h->rhoy ^ 1));
     void* const out[4] = {F(h, F_SX), F(h, F_SY), F(h, F_SZ), F(h, F_DHAT)};
     // with D live the pass also forms |grad rho|^2 for the nonlinear pass's
     // pressure term, in the D field (read by no stage; formed on request)
@@ -897,14 +928,21 @@ int lapgrad_output(qsm_handle_s* h) {
   return QSM_OK;
 }
 
-// The same for grad rho (component c: DST-III along c).
+// The same for grad rho: r_x, r_y by the z passes in place; r_z (which the
+// stress pass formed in registers only) by the z pass of the z gradient
+// operator on the y-passed field of the rho before the latest, as the stress
+// pass read it.
 int gradrho_output(qsm_handle_s* h) {
   if (!h->r_yspec) return QSM_OK;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < 2; ++c) {
     void* f = F(h, F_RX + c);
-    const int rc = run_pass(h, 2, true, c == 2, plain_load(f), plain_store(f), false, nullptr);
+    const int rc = run_pass(h, 2, true, false, plain_load(f), plain_store(f), false, nullptr);
     if (rc) return rc;
   }
+  LoadOp L{};
+  L.code = LD_GRAD_Z; L.axis = 2; L.a[0] = F(h, F_RHOY0 + (h->rhoy ^ 1));
+  const int rc = run_pass(h, 2, true, true, L, plain_store(F(h, F_RZ)), true, nullptr);
+  if (rc) return rc;
   h->r_yspec = false;
   return QSM_OK;
 }
@@ -1159,6 +1197,7 @@ extern "C" int qsm_set_psi(qsm_handle h, const double* psi) {
   for (int f = 0; f < F_COUNT; ++f) HIPCHK(hipMemsetAsync(h->f[f], 0, n * h->esz, h->stream));
   HIPCHK(hipMemsetAsync(h->dsc, 0, D_COUNT * 8, h->stream));
   h->psi_cur = F_PSI_A; h->psi_old = F_PSI_B; h->rho_cur = F_RHO_A; h->rho_old = F_RHO_B;
+  h->rhoy = 0;  // rhohat = 0: its passed fields are the zeros set above
   // psi (FP64 host) -> psihat = scale DCT(psi) in FP64, whatever the precision
   HIPCHK(hipMemcpyAsync(h->psihat_old, psi, n * 8, hipMemcpyHostToDevice, h->stream));
   // forward FP64 DCT of psi: z, y, x passes on psihat; scale in the last store
@@ -1216,7 +1255,8 @@ std::vector<long long> step_key(const qsm_handle_s* h) {
   const qsm_params& p = h->p;
   const bool quasi = p.variant == QSM_QUASI;
   return {(long long)h->psi_cur, (long long)h->rho_cur, (long long)(uintptr_t)h->psihat,
-          n == 1, n >= 2, quasi && n > p.divv_switch, quasi && n > p.divv_switch + 1};
+          (long long)h->rhoy, n == 1, n >= 2, quasi && n > p.divv_switch,
+          quasi && n > p.divv_switch + 1};
 }
 
 int one_step(qsm_handle_s* h) {
@@ -1996,7 +2036,10 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   if (!ok) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is truncated", path);
   if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)h->n, h->stream));
   if (h->p.variant == QSM_QUASI) {
-    const int rc = psi_xy(h);  // psi's passed fields, by the passes the step runs
+    // psi's and rho's passed fields, by the passes the step runs
+    int rc = psi_xy(h);
+    h->rhoy = 0;
+    if (!rc) rc = rho_inverse(h, nullptr);
     if (rc) return rc;
   }
   h->step = hd.step;

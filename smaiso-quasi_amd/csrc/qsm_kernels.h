@@ -68,7 +68,6 @@ long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 // caches.
 enum GroupCode : int {
   GRP_PSI = 0,     // F, G, grad psi x, lap grad psi (x, y, z) from psihat
-  GRP_RHO,         // grad rho (x, y, z) from rhohat
   GRP_PROJ,        // p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
   GRP_STB_PSI,     // sh, grad psi (x, y, z)
   GRP_VEL,         // v_x, v_y, v_z (stb, and quasi steps whose p is read by nothing)
@@ -93,7 +92,9 @@ struct GroupArgs {
 // operator (LD_STRESS3, LD_STRESS_STB3, LD_STRESS3_DIVV), out[c] receives
 // value c, dmask bit c selects DST along z for value c.  The quasi stress
 // operators take lap grad psi y-passed (lz = 1), and with the divergence
-// grad rho too (lz = 2): the pass completes their inverse transforms itself.
+// grad rho too (lz = 2: r_x, r_y y-passed in a[13], a[14], and rho's
+// y-passed field in a[15], from which the pass forms r_z): the pass
+// completes their inverse transforms itself.
 struct MultiArgs {
   Geo geo;
   Phys phys;
@@ -104,6 +105,7 @@ struct MultiArgs {
   int lz;          // 1: a[6..8] (lap grad psi) are y-passed, 2: a[13..15] (grad rho) too;
                    // the pass runs their inverse z rounds
   void* aux;       // lz = 2: |grad rho|^2 of each element (null: not formed)
+  const double* sz;  // lz = 2: the s_z wavenumbers (device)
 };
 hipError_t launch_multi(int prec_fp32, int n, const MultiArgs& A, long* nblocks, hipStream_t s);
 bool multi_supported(int prec_fp32, int n);

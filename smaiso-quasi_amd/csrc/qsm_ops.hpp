@@ -65,7 +65,8 @@ enum LoadCode : int {
   LD_STRESS3,       // LD_STRESS for c = x, y, z; a: rho, G, F, g_x,y,z, l_x,y,z
   LD_STRESS_STB3,   // LD_STRESS_STB for c = x, y, z; a: sh, psi, g_x,y,z
   LD_STRESS3_DIVV,  // LD_STRESS3, then LD_DIVV as a fourth value; a: as LD_STRESS3, then
-                    //   rho_old, v_x,y,z, r_x,y,z (rho is a0)
+                    //   rho_old, v_x,y,z, r_x,y,z (rho is a0; in the multi-output pass
+                    //   with lz = 2, a15 is rho's y-passed field, r_z formed from it)
   LD_NONLIN_PY,     // LD_NONLIN_P with p (a10) y-passed and |grad rho|^2 (a11) formed by
                     //   the stress pass: the pass completes p's inverse z transform and
                     //   keeps the pressure term

@@ -38,7 +38,6 @@ static int list_codes(OpList<LDs...>, int* codes) {
 int group_ops(int group, int* codes) {
   switch (group) {
     case GRP_PSI: return list_codes(GroupOps<GRP_PSI>::L{}, codes);
-    case GRP_RHO: return list_codes(GroupOps<GRP_RHO>::L{}, codes);
     case GRP_PROJ: return list_codes(GroupOps<GRP_PROJ>::L{}, codes);
     case GRP_STB_PSI: return list_codes(GroupOps<GRP_STB_PSI>::L{}, codes);
     case GRP_VEL: return list_codes(GroupOps<GRP_VEL>::L{}, codes);
@@ -49,7 +48,7 @@ int group_ops(int group, int* codes) {
 bool group_inverse(int group) {
   switch (group) {
 #define QSM_G(GG) case GG: return GroupOps<GG>::INV;
-    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_VEL)
+    QSM_G(GRP_PSI) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_VEL)
 #undef QSM_G
     default: return false;
   }
@@ -71,7 +70,7 @@ bool multi_supported(int prec_fp32, int n) {
 int group_axis(int group) {
   switch (group) {
 #define QSM_G(GG) case GG: return GroupOps<GG>::AX;
-    QSM_G(GRP_PSI) QSM_G(GRP_RHO) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_VEL)
+    QSM_G(GRP_PSI) QSM_G(GRP_PROJ) QSM_G(GRP_STB_PSI) QSM_G(GRP_VEL)
 #undef QSM_G
     default: return -1;
   }

@@ -740,6 +740,28 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
               held[PER - 1] = xv;
             }
           }
+        } else if constexpr (LD == LD_GRAD_X + AX && SPLIT) {
+          // the gradient along the pass axis (of a field transformed along
+          // the axes before it): the shifted reads at the walk's split
+          // address plus a uniform offset (inside the field or its padding),
+          // then the operator's expression as in load_eval
+          const int o = AX == kAxX ? G.ny * G.nz : (AX == kAxY ? G.nz : 1);
+          const int nc = AX == kAxX ? G.nx : (AX == kAxY ? G.ny : G.nz);
+          T raw[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) raw[c] = ld_at<T>(Lop.a[0], tg.wbase, walk_uni(s0 + c) + o);
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            int q, n, i, j, k;
+            Ix idx;
+            bool ok;
+            tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
+            const Kv K = kv_at(i, j, k, q);
+            const int m = AX == kAxX ? i : (AX == kAxY ? j : k);
+            const double st = AX == kAxX ? K.sx : (AX == kAxY ? K.sy : K.sz);
+            if constexpr (sizeof(T) == sizeof(float)) xs[c] = m < nc - 1 ? (T)(-st) * raw[c] : (T)0;
+            else xs[c] = (T)(-st * (m < nc - 1 ? (double)raw[c] : 0.0));
+          }
         } else {
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
@@ -879,15 +901,18 @@ __host__ __device__ inline long multi_tiles(const Geo& g) {
 
 // One inverse z round of a multi-output tile: field a[F_g] (F_g < 0: zero)
 // in pencil group g, DST-III for group DG; each thread keeps the outputs of
-// its walk elements, group g at lz[s][BASE + g].  Every load of the round is
-// issued before the first LDS write, so they are in flight together.
+// its walk elements, group g at lz[s][BASE + g].  Group GZ (>= 0) takes the
+// z gradient of its field instead, -s_z[k] a[k+1] (0 at k = nz - 1; the
+// LD_GRAD_Z operator): the field then holds a y-passed scalar whose z
+// derivative the round transforms.  Every load of the round is issued
+// before the first LDS write, so they are in flight together.
 template <typename T, int N, int R, int TKP, int BLOCK, int M, int E, int R1, class RADICES>
 struct MultiRound {
   static constexpr int PER = R * N / BLOCK;
-  template <int F0, int F1, int F2, int F3, int DG, int BASE, class WI>
-  __device__ __forceinline__ static void run(const LoadOp& L, const WI& walk_idx, T* rbuf,
-                                             cx<T>* buf, const cx<T>* tp, const cx<T>* ts,
-                                             int p, int t, int tid, T (*lz)[6]) {
+  template <int F0, int F1, int F2, int F3, int DG, int BASE, int GZ, class WI, class WK>
+  __device__ __forceinline__ static void run(const LoadOp& L, const WI& walk_idx, const WK& gz_of,
+                                             T* rbuf, cx<T>* buf, const cx<T>* tp,
+                                             const cx<T>* ts, int p, int t, int tid, T (*lz)[6]) {
     constexpr int F[4] = {F0, F1, F2, F3};
     {
       T x[PER][4];
@@ -895,7 +920,16 @@ struct MultiRound {
       for (int s = 0; s < PER; ++s) {
         const Ix idx = walk_idx(s);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) x[s][g] = F[g] >= 0 ? ld<T>(L.a[F[g] >= 0 ? F[g] : 0], idx) : (T)0;
+        for (int g = 0; g < 4; ++g) {
+          if constexpr (GZ >= 0) {
+            if (g == GZ) {
+              // inside the field or its padding (one y-z slab past the end)
+              x[s][g] = gz_of(s, ld<T>(L.a[F[GZ]], idx + 1));
+              continue;
+            }
+          }
+          x[s][g] = F[g] >= 0 ? ld<T>(L.a[F[g] >= 0 ? F[g] : 0], idx) : (T)0;
+        }
       }
       WalkPos<kAxZ, N, R, TKP, BLOCK, 0> wn;  // natural order (DCT-III)
       WalkPos<kAxZ, N, R, TKP, BLOCK, 2> wr;  // reversed (DST-III)
@@ -937,9 +971,11 @@ struct MultiRound {
 // pencil groups 0-2, one inverse transform runs on all TK pencils (DST-III
 // for l_z, by reversed positions and signs as in pass_tile), and each thread
 // keeps the real-space values of its walk elements in registers for the
-// operator.  LZ = 2 (with the divergence): grad rho a[13..15] as well, r_x in
-// group 3 of that round and r_y, r_z (DST-III) in groups 0-1 of a second.
-// Unused groups are zero.  The separate inverse z passes, a read and a write
+// operator.  LZ = 2 (with the divergence): grad rho as well, r_x (a[13],
+// y-passed) in group 3 of that round and r_y (a[14], y-passed) and r_z
+// (DST-III) in groups 0-1 of a second.  r_z is formed in the round from
+// rho's y-passed field a[15] (the z gradient operator), so it has no
+// transform of its own.  Unused groups are zero.  The separate inverse z passes, a read and a write
 // of each field, are then not run.
 template <typename T, int N, int LD, int LZ, class SM>
 __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int tile, SM& sm,
@@ -985,10 +1021,33 @@ __device__ __forceinline__ void pass_tile_multi(const MultiArgs& A, const int ti
   T lz[LZ ? PER : 1][6];
   if constexpr (LZ != 0) {
     using IR = MultiRound<T, N, R, TKP, BLOCK, M, E, R1, typename PL::R>;
-    IR::template run<6, 7, 8, (LZ == 2 ? 13 : -1), 2, 0>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t,
-                                                        tid, lz);
+    // the z gradient operator at walk element s (the expression of
+    // LD_GRAD_Z in load_eval) of the value v read at k + 1.  Split walks
+    // repeat the same RZ columns k, so their -s_z[k] are read once.
+    constexpr int KZ = SPLIT ? RZ : PER;
+    double nsz[LZ == 2 ? KZ : 1];
+    bool okz[LZ == 2 ? KZ : 1];
     if constexpr (LZ == 2) {
-      IR::template run<14, 15, -1, -1, 1, 4>(A.ld, walk_idx, rbuf, buf, tp, ts, p, t, tid, lz);
+#pragma unroll
+      for (int u = 0; u < KZ; ++u) {
+        int q, n, i, j, k;
+        Ix idx;
+        bool ok;
+        tg.walk(tid, u, q, n, i, j, k, idx, ok);
+        nsz[u] = -A.sz[k];
+        okz[u] = k < A.geo.nz - 1;
+      }
+    }
+    auto gz_of = [&](int s, T v) -> T {
+      const int u = LZ == 2 ? s % KZ : 0;
+      if constexpr (sizeof(T) == sizeof(float)) return okz[u] ? (T)nsz[u] * v : (T)0;
+      else return (T)(nsz[u] * (okz[u] ? (double)v : 0.0));
+    };
+    IR::template run<6, 7, 8, (LZ == 2 ? 13 : -1), 2, 0, -1>(A.ld, walk_idx, gz_of, rbuf, buf, tp, ts,
+                                                            p, t, tid, lz);
+    if constexpr (LZ == 2) {
+      IR::template run<14, 15, -1, -1, 1, 4, 1>(A.ld, walk_idx, gz_of, rbuf, buf, tp, ts, p, t, tid,
+                                               lz);
       // |grad rho|^2 for the nonlinear pass's pressure term (the expression
       // of LD_NONLIN_P), so that pass need not transform grad rho again
       if (A.aux) {
@@ -1241,16 +1300,6 @@ struct GroupOps<GRP_PSI> {
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
   static constexpr int PARTS = 2;
-  static constexpr int VAR64 = 1;
-  static constexpr bool FUSED = false;
-};
-template <>
-struct GroupOps<GRP_RHO> {
-  using L = OpList<LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
-  static constexpr bool INV = true;
-  static constexpr int AX = kAxX;
-  static constexpr bool XORD = false;
-  static constexpr int PARTS = 1;  // 3 parts: 656 -> 710 us
   static constexpr int VAR64 = 1;
   static constexpr bool FUSED = false;
 };
@@ -1522,7 +1571,6 @@ template <typename T, int N, int AX>
 hipError_t launch_group_n(int group, const GroupArgs& G, long* nblocks, hipStream_t s) {
   switch (group) {
     case GRP_PSI: return launch_group_g<T, N, AX, GRP_PSI>(G, nblocks, s);
-    case GRP_RHO: return launch_group_g<T, N, AX, GRP_RHO>(G, nblocks, s);
     case GRP_PROJ: return launch_group_g<T, N, AX, GRP_PROJ>(G, nblocks, s);
     case GRP_STB_PSI: return launch_group_g<T, N, AX, GRP_STB_PSI>(G, nblocks, s);
     case GRP_VEL: return launch_group_g<T, N, AX, GRP_VEL>(G, nblocks, s);
