@@ -446,14 +446,17 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
   }
   switch (S.code) {
     case ST_PLAIN: case ST_FILTER: wr = e; break;
-    case ST_UPDATE: case ST_UPDATE_STB: rd += 16 + e; wr = 8 + e; break;
+    case ST_FILTER_X: wr = 2 * e; break;
+    case ST_UPDATE: case ST_UPDATE_STB: rd += 16 + e; wr = 8 + e + (h->fp32 ? e : 0); break;
+    case ST_UPDATE_X: rd += 16 + e; wr = 8 + 2 * e + (h->fp32 ? e : 0); break;
     default: wr = e;
   }
   return (rd + wr) * n;
 }
 
+// var >= 0: the tile variant of a plain pass instead of the tuned one
 int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, const StoreOp& S,
-             bool first_pass, double* part) {
+             bool first_pass, double* part, int var_force = -1) {
   PassArgs A{};
   A.geo = h->geo;
   A.waves = h->waves;
@@ -467,7 +470,9 @@ int run_pass(qsm_handle_s* h, int axis, bool inv, bool dst, const LoadOp& L, con
   if (part && (inv || axis != 2 || !dc_capable(L.code)))
     return fail(QSM_ERR_STATE, "pass: FP64 mean-mode sum requested for a pass that has none");
   if (h->dry) return QSM_OK;
-  const int var = (L.code == LD_PLAIN && S.code == ST_PLAIN) ? h->variant[axis][inv ? 1 : 0] : 0;
+  const int var = (L.code == LD_PLAIN && S.code == ST_PLAIN)
+                      ? (var_force >= 0 ? var_force : h->variant[axis][inv ? 1 : 0])
+                      : 0;
   if (h->rec) {
     PassRec r{};
     r.kind = 0; r.axis = axis; r.n = n; r.inv = inv; r.ld = L.code; r.st = S.code; r.A = A;
@@ -646,18 +651,24 @@ const void* psihat_src(const qsm_handle_s* h) {
 }
 
 // psi's x- and y-passed fields from psihat (quasi): the first two passes of
-// psi = IDCT(psihat), kept for the transforms of grad psi y and z.
-int psi_xy(qsm_handle_s* h) {
-  int rc = run_pass(h, 0, true, false, plain_load(psihat_src(h)), plain_store(F(h, F_PSIX)), true, nullptr);
+// psi = IDCT(psihat), kept for the transforms of grad psi y and z.  The step
+// forms the x-passed field in its update pass (ST_UPDATE_X), by the default
+// radix plan; the x pass here (set-up, checkpoint read) takes the same plan
+// (variant 0), so either way the field has the same bits.  with_x = false:
+// the x-passed field is already there (the update pass wrote it).
+int psi_xy(qsm_handle_s* h, bool with_x = true) {
+  int rc = with_x ? run_pass(h, 0, true, false, plain_load(psihat_src(h)), plain_store(F(h, F_PSIX)),
+                             true, nullptr, 0)
+                  : QSM_OK;
   if (!rc) rc = run_pass(h, 1, true, false, plain_load(F(h, F_PSIX)), plain_store(F(h, F_PSIY)), false, nullptr);
   return rc;
 }
 
 // psi = IDCT(psihat) into out (quasi.cpp:924-927, 1961); the quasi variant
 // passes through F_PSIX and F_PSIY, which the next step reads.
-int psi_inverse(qsm_handle_s* h, void* out) {
+int psi_inverse(qsm_handle_s* h, void* out, bool with_x = true) {
   if (h->p.variant != QSM_QUASI) return inv3(h, -1, plain_load(psihat_src(h)), out);
-  const int rc = psi_xy(h);
+  const int rc = psi_xy(h, with_x);
   return rc ? rc : run_pass(h, 2, true, false, plain_load(F(h, F_PSIY)), plain_store(out), false, nullptr);
 }
 
@@ -677,10 +688,14 @@ int grad_psi_yz(qsm_handle_s* h) {
 
 // rho = IDCT(rhohat) into out (quasi.cpp:1263-1268) through F_RHOX and the
 // free one of the two y-passed fields, which then holds the latest rho's.
-int rho_inverse(qsm_handle_s* h, void* out) {
+// The step's density filter pass forms F_RHOX itself (ST_FILTER_X, with_x =
+// false); the x pass here takes the same radix plan (variant 0).
+int rho_inverse(qsm_handle_s* h, void* out, bool with_x = true) {
   const int cur = h->rhoy ^ 1;
   void* y = F(h, F_RHOY0 + cur);
-  int rc = run_pass(h, 0, true, false, plain_load(F(h, F_RHOHAT)), plain_store(F(h, F_RHOX)), true, nullptr);
+  int rc = with_x ? run_pass(h, 0, true, false, plain_load(F(h, F_RHOHAT)), plain_store(F(h, F_RHOX)),
+                             true, nullptr, 0)
+                  : QSM_OK;
   if (!rc) rc = run_pass(h, 1, true, false, plain_load(F(h, F_RHOX)), plain_store(y), false, nullptr);
   if (!rc && out) rc = run_pass(h, 2, true, false, plain_load(y), plain_store(out), false, nullptr);
   if (!rc) h->rhoy = cur;
@@ -793,10 +808,11 @@ int quasi_step(qsm_handle_s* h) {
   // S3: density law + filter; rho (quasi.cpp:1237-1268)
   L = LoadOp{}; L.code = LD_RHOSTAR;
   L.a[0] = psi; L.a[1] = F(h, F_GX); L.a[2] = F(h, F_GY); L.a[3] = F(h, F_GZ);
-  StoreOp S{}; S.code = ST_FILTER; S.d[0] = F(h, F_RHOHAT);
+  // the filter pass also runs rho's inverse x pass (into F_RHOX)
+  StoreOp S{}; S.code = ST_FILTER_X; S.d[0] = F(h, F_RHOHAT); S.d[3] = F(h, F_RHOX);
   if (mixed) { S.dc = h->dsc + D_RHOSUM; S.dc_out = h->dsc + D_RHOHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_RHOHAT), S, mixed ? D_RHOSUM : -1))) return rc;
-  if ((rc = rho_inverse(h, rho))) return rc;
+  if ((rc = rho_inverse(h, rho, false))) return rc;
 
   // S4: divergence (quasi.cpp:1273-1293).  Up to divv_switch the reference
   // transforms an all-zero D; D-hat = 0 then, which the D-hat array already
@@ -873,14 +889,16 @@ int quasi_step(qsm_handle_s* h) {
   L.a[7] = F(h, F_GX); L.a[8] = F(h, F_GY); L.a[9] = F(h, F_GZ);
   L.a[10] = F(h, F_P); L.a[11] = F(h, F_RX); L.a[12] = F(h, F_RY); L.a[13] = F(h, F_RZ);
   if (L.code == LD_NONLIN_PY) L.a[11] = F(h, F_DIVV);  // |grad rho|^2 from the stress pass
-  S = StoreOp{}; S.code = ST_UPDATE; S.first = n == 1;
+  // the update pass also runs psi's inverse x pass (into F_PSIX)
+  S = StoreOp{}; S.code = ST_UPDATE_X; S.first = n == 1;
   S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.d[2] = h->psihat32; S.s[0] = h->psihat;
+  S.d[3] = F(h, F_PSIX);
   if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
   std::swap(h->psihat, h->psihat_old);
 
   // psi = IDCT(psihat) into the psi_old buffer, then swap (psi_old <- psi)
-  if ((rc = psi_inverse(h, F(h, h->psi_old)))) return rc;
+  if ((rc = psi_inverse(h, F(h, h->psi_old), false))) return rc;
   std::swap(h->psi_cur, h->psi_old);
   return QSM_OK;
 }

@@ -77,7 +77,19 @@ enum StoreCode : int {
   ST_FILTER,        // d0[idx] = scale v exp(-filter K^2)                 quasi.cpp:1260-1261
   ST_UPDATE,        // semi-implicit AB2/CN update of psihat              quasi.cpp:1949-1951
   ST_UPDATE_STB,    // CN update of psihat (stb)                          quasi-stb.cpp:1954-1956
+  // The same stores, then the inverse x transform (DCT-III) of the stored
+  // spectral values into d3: the field's first inverse pass, run on the tile
+  // the forward x pass holds (rho's x-passed field after the density filter;
+  // psi's after the update, from psihat_new, its FP32 copy in mixed precision)
+  ST_FILTER_X,
+  ST_UPDATE_X,
 };
+
+// the store operator an X code extends, and whether a code has the x inverse
+__host__ __device__ constexpr int st_base(int st) {
+  return st == ST_FILTER_X ? ST_FILTER : (st == ST_UPDATE_X ? ST_UPDATE : st);
+}
+__host__ __device__ constexpr bool st_inverts(int st) { return st == ST_FILTER_X || st == ST_UPDATE_X; }
 
 struct LoadOp {
   int code;
@@ -90,7 +102,7 @@ struct LoadOp {
 struct StoreOp {
   int code;
   int first;           // UPDATE: first step (N_old = N)
-  void* d[3];          // outputs
+  void* d[4];          // outputs (d3: the x-inverse output of ST_FILTER_X / ST_UPDATE_X)
   const void* s[2];    // extra inputs
   const double* dc;    // mixed precision: FP64 sum of the forward input (DC override), or null
   double* dc_out;      // mixed precision: FP64 (0,0,0) mode written here, or null
@@ -163,16 +175,18 @@ struct Kv {
   double qx, sx, fx, qy, sy, fy, qz, sz, fz;
 };
 
-template <int LD, int ST>
+template <int LD, int STX>
 constexpr bool uses_kv() {
+  constexpr int ST = st_base(STX);
   return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
          LD == LD_VEL0 || LD == LD_VEL1 || LD == LD_VEL2 || (LD >= LD_GRAD_X && LD <= LD_LAPGRAD_Z) ||
          ST == ST_UPDATE || ST == ST_UPDATE_STB || ST == ST_FILTER;
 }
 
 // Which wavenumbers of axis c an operator reads: q_c, s_c, f_c.
-template <int LD, int ST>
+template <int LD, int STX>
 constexpr bool uses_q(int c) {
+  constexpr int ST = st_base(STX);
   return LD == LD_COEF_F || LD == LD_COEF_G || LD == LD_SH || LD == LD_PRESSURE ||
          ST == ST_UPDATE || ST == ST_UPDATE_STB ||
          (LD == LD_VEL0 && c != 0) || (LD == LD_VEL1 && c != 1) || (LD == LD_VEL2 && c != 2) ||
@@ -186,7 +200,7 @@ constexpr bool uses_s(int c) {
 }
 template <int LD, int ST>
 constexpr bool uses_f(int) {
-  return ST == ST_FILTER;
+  return st_base(ST) == ST_FILTER;
 }
 // pass-axis tables staged for an operator (bit 0: q, 1: s, 2: f)
 template <int AX, int LD, int ST>
@@ -576,13 +590,15 @@ struct StIn {
   double ph, pho, nold;
 };
 
-template <int ST>
+template <int STX>
 constexpr bool store_reads() {
+  constexpr int ST = st_base(STX);
   return ST == ST_UPDATE || ST == ST_UPDATE_STB || ST == ST_FILTER;
 }
 
-template <typename T, int ST>
+template <typename T, int STX>
 __device__ __forceinline__ StIn store_inputs(const StoreOp& S, const Kv& K, Ix idx) {
+  constexpr int ST = st_base(STX);
   StIn in{0.0, 0.0, 0.0};
   if constexpr (ST == ST_UPDATE || ST == ST_UPDATE_STB) {
     in.ph = ld<double>(S.s[0], idx);
@@ -609,9 +625,13 @@ __device__ __forceinline__ DcIn dc_in(const StoreOp& S) {
 // s0: psihat (double, in); mixed precision (T = float): d2 receives psihat_new
 // rounded to FP32, the copy the next step's spectral prologues read.
 // quasi.cpp:1949-1951, quasi-stb.cpp:1954-1956.
-template <typename T, int ST>
-__device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, const Kv& K, Ix idx,
-                                            T v, const StIn& in, const DcIn& D) {
+// Returns the stored spectral value in the field precision (the filtered
+// density; psihat_new, rounded to FP32 in mixed precision), the input of the
+// x inverse of the X codes.
+template <typename T, int STX>
+__device__ __forceinline__ T store_apply(const StoreOp& S, const Phys& P, const Kv& K, Ix idx,
+                                         T v, const StIn& in, const DcIn& D) {
+  constexpr int ST = st_base(STX);
   const bool z = idx == 0;
   if constexpr (ST == ST_FILTER) {
     // quasi.cpp:1260-1261 with exp(-filter K^2) = fx[i] fy[j] fz[k] (in.ph)
@@ -619,7 +639,7 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
     const double r = P.scale * raw * in.ph;
     st<T>(S.d[0], idx, (T)r);
     if (z && S.dc_out) S.dc_out[0] = r;
-    return;
+    return (T)r;
   }
   const double mq2 = sq(K.qx) + sq(K.qy) + sq(K.qz);
   const double alin = P.eps - P.alpha * sq(P.q02 - mq2);
@@ -637,6 +657,7 @@ __device__ __forceinline__ void store_apply(const StoreOp& S, const Phys& P, con
   st<T>(S.d[1], idx, (T)nh);
   if constexpr (sizeof(T) < sizeof(double)) st<T>(S.d[2], idx, (T)nw);
   if (z && S.dc_out) S.dc_out[0] = nh;
+  return (T)nw;
 }
 
 // Load operators whose forward transform keeps its (0,0,0) mode in FP64 in

@@ -284,8 +284,9 @@ constexpr int smem_tables() {
 // pre-processing twiddles too: their LDS takes the inverse layout (2M
 // entries, more than the forward post-processing's 3(M/2+1)).
 template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
-using SmemFor = Smem<T, N, AX, INV || pre_rounds(LD), V,
-                     smem_tables<T, N, AX, INV || pre_rounds(LD), V, kv_tables<AX, LD, ST>()>()>;
+using SmemFor = Smem<T, N, AX, INV || pre_rounds(LD) || st_inverts(ST), V,
+                     smem_tables<T, N, AX, INV || pre_rounds(LD) || st_inverts(ST), V,
+                                 kv_tables<AX, LD, ST>()>()>;
 
 // DCT-III pre-processing of pencil p straight into the stage-1 inputs of
 // thread t (the real samples X of the pencil in LDS, natural order):
@@ -364,8 +365,10 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
   constexpr int R1 = PL::R1;
   using TW = TwLayout<M>;
   constexpr int NTP = INV ? TW::NPRE : TW::NPOST;
-  static_assert(SM::NTP == (INV || pre_rounds(LD) ? TW::NPRE : NTP) && SM::TKP == TKP,
+  static_assert(SM::NTP == (INV || pre_rounds(LD) || st_inverts(ST) ? TW::NPRE : NTP) &&
+                    SM::TKP == TKP,
                 "LDS layout of another pass");
+  static_assert(!st_inverts(ST) || (AX == kAxX && !INV), "x inverse after the store: forward x passes");
   static_assert(!pre_rounds(LD) || (AX == kAxZ && !INV), "pre-rounds: forward z passes");
   cx<T>* const buf = sm.buf;
   cx<T>* const ts = sm.ts;
@@ -685,9 +688,58 @@ __device__ __forceinline__ void pass_tile(const Geo& G, const Waves& W, const Ph
             bool ok;
             tg.walk(tid, s0 + c, q, n, i, j, k, idx, ok);
             if (!ok) continue;
-            const T y = rbuf[wp.pos(tid, s0 + c, oddn)];
-            if constexpr (store_reads<ST>()) store_apply<T, ST>(Sop, Ph, kv_at(i, j, k, q), idx, y, in[c], dcv);
-            else store_eval<T, ST>(Sop, idx, y);
+            const int lp = wp.pos(tid, s0 + c, oddn);
+            const T y = rbuf[lp];
+            if constexpr (store_reads<ST>()) {
+              const T sv = store_apply<T, ST>(Sop, Ph, kv_at(i, j, k, q), idx, y, in[c], dcv);
+              // the stored value at this element's natural-order slot (read
+              // and rewritten by this thread only), the x inverse's input
+              if constexpr (st_inverts(ST)) rbuf[lp] = sv;
+            } else {
+              store_eval<T, ST>(Sop, idx, y);
+            }
+          }
+        }
+      }
+    }
+    if constexpr (st_inverts(ST)) {
+      // ---- the inverse x transform (DCT-III) of the stored values into d3:
+      // the pre-processing twiddles replace the post-processing ones, then
+      // the same steps and register stores as an inverse x pass
+      __syncthreads();
+      {
+        const cx<T>* __restrict__ tw = (const cx<T>*)twp;
+        for (int u = tid; u < TW::NPRE; u += BLOCK) tp[u] = tw[TW::PRE + u];
+      }
+      __syncthreads();
+      cx<T> vi[E];
+      inv_preprocess<T, N, M, E, TKP, R1>(vi, rbuf, tp, p, t);
+      run_fft<T, N, M, E, TKP, 1, false>(typename PL::R{}, vi, buf, p, t, ts);
+      constexpr int RL = Last<typename PL::R>::value, NBL = E / RL;
+      if (tg.colok) {
+        const Ix g4u = tg.cbase + (Ix)(4 * t) * tg.axs, g4d = tg.cbase - (Ix)(4 * t) * tg.axs;
+        const Ix tu = g4u, td = tg.cbase + (Ix)(N - 1 - 4 * t) * tg.axs;
+        const int ax = (int)tg.axs;
+#pragma unroll
+        for (int b = 0; b < NBL; ++b) {
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const int C = b * TP + r * (M / RL);
+            const cx<T> zv = vi[b * RL + r];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              const int u0 = 2 * C + c;
+              const T y = c ? zv.y : zv.x;
+              if (2 * (TP - 1) + u0 < M) {
+                st_at<T>(Sop.d[3], tu, 2 * u0 * ax, y);
+              } else if (u0 >= M) {
+                st_at<T>(Sop.d[3], td, (N - 2 * u0) * ax, y);
+              } else {
+                const Ix iu = g4u + (Ix)(2 * u0) * tg.axs;
+                const Ix idn = g4d + (Ix)(2 * N - 1 - 2 * u0) * tg.axs;
+                st<T>(Sop.d[3], 2 * t + u0 < M ? iu : idn, y);
+              }
+            }
           }
         }
       }
@@ -1222,6 +1274,24 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
       tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
 }
 
+// Forward x passes that also run the x inverse of their stored values
+// (ST_FILTER_X, ST_UPDATE_X): the register allocator is held to the two
+// workgroups per CU that the LDS allows (left alone it takes 132 VGPRs in
+// FP32 at 512, one workgroup per CU).
+template <typename T, int N, int V>
+constexpr int inv_store_waves() {
+  constexpr int w = (TileCfg<T, N, kAxX, V>::BLOCK + 255) / 256;  // waves per SIMD per workgroup
+  return 2 * w > 8 ? 8 : 2 * w;
+}
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
+__global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(inv_store_waves<T, N, V>()))) k_pass_sx(const PassArgs A) {
+  __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
+  pass_tile<T, N, AX, INV, LD, ST, V>(
+      A.geo, A.waves, A.phys, A.ld, A.st, A.tw, A.part, A.dst,
+      tile_order<T, N, AX, V>(A.geo, (int)blockIdx.x, (int)gridDim.x), sm);
+}
+
 // ---- software-pipelined plain inverse passes (variants 4 and 5).  Where a tile
 // fills a CU's LDS alone (FP64 x passes at N = 512: 256-byte rows of 32
 // columns, 135 KB), one workgroup per CU runs its load, transform and store
@@ -1591,7 +1661,10 @@ hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const
   const long nb = tiles_of<T, N, AX, V>(A.geo);
   if (nblocks_out) *nblocks_out = nb;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_pass<T, N, AX, INV, LD, ST, V>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
+  if constexpr (st_inverts(ST))
+    hipLaunchKernelGGL((k_pass_sx<T, N, AX, INV, LD, ST, V>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
+  else
+    hipLaunchKernelGGL((k_pass<T, N, AX, INV, LD, ST, V>), dim3((unsigned)nb), dim3(C::BLOCK), 0, s, A);
   return hipGetLastError();
 }
 
@@ -1683,7 +1756,7 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
     if (!inv && ld == LD_PLAIN) {
       switch (st) {
 #define QSM_S(OP) case OP: return launch_one<T, N, AX, false, LD_PLAIN, OP>(A, nb, s, B);
-        QSM_S(ST_FILTER) QSM_S(ST_UPDATE) QSM_S(ST_UPDATE_STB)
+        QSM_S(ST_FILTER) QSM_S(ST_UPDATE) QSM_S(ST_UPDATE_STB) QSM_S(ST_FILTER_X) QSM_S(ST_UPDATE_X)
 #undef QSM_S
         default: break;
       }
