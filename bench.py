@@ -100,7 +100,9 @@ def kernel_symbol(k, shape, fp32):
     if k["variant"] >= 4:  # software-pipelined persistent plain pass
         return (f"void qsm::k_pass_pipe<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
                 f"{k['variant']}>(qsm::PassArgs, int)")
-    return (f"void qsm::k_pass<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+    # forward x passes that also run the x inverse of what they store
+    kern = "k_pass_sx" if STORE_OPS[k["st"]] in ("filter_x", "update_x") else "k_pass"
+    return (f"void qsm::{kern}<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
             f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
 
 
