@@ -101,7 +101,7 @@ def kernel_symbol(k, shape, fp32):
         return (f"void qsm::k_pass_pipe<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
                 f"{k['variant']}>(qsm::PassArgs, int)")
     # forward x passes that also run the x inverse of what they store
-    kern = "k_pass_sx" if STORE_OPS[k["st"]] in ("filter_x", "update_x") else "k_pass"
+    kern = "k_pass_sx" if STORE_OPS[k["st"]] in ("filter_x", "update_x", "update_stb_x") else "k_pass"
     return (f"void qsm::{kern}<{t}, {n_ax}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
             f"{k['ld']}, {k['st']}, {k['variant']}>(qsm::PassArgs)")
 

@@ -329,11 +329,11 @@ enum Fld {
   F_NHAT_OLD, F_NTMP, F_RHOHAT, F_RHO_A, F_RHO_B, F_VX, F_VY, F_VZ, F_PSI_A, F_PSI_B,
   F_F, F_G, F_GX, F_GY, F_GZ, F_RX, F_RY, F_RZ, F_LX, F_LY, F_LZ, F_DHAT, F_DIVV,
   F_SX, F_SY, F_SZ, F_P,
-  // psi's x-passed and y-passed fields (quasi): the transforms of grad psi y
-  // and z start from them.  rho's x-passed field and its y-passed fields
+  // psi's x-passed and y-passed fields: the transforms of grad psi y and z
+  // start from them.  rho's x-passed field and its y-passed fields
   // (ping-pong: the latest rho's and the one before, which the lagged grad
-  // rho reads): the same for grad rho y and z.  Derived from psihat and
-  // rhohat; not in checkpoints.
+  // rho reads): the same for grad rho y and z (quasi).  Derived from psihat
+  // and rhohat; not in checkpoints.
   F_PSIX, F_PSIY, F_RHOX, F_RHOY0, F_RHOY1, F_COUNT
 };
 constexpr int F_CKPT = F_PSIX;  // fields a checkpoint holds: those before F_PSIX
@@ -448,7 +448,7 @@ double op_bytes(const qsm_handle_s* h, const LoadOp& L, const StoreOp& S, bool f
     case ST_PLAIN: case ST_FILTER: wr = e; break;
     case ST_FILTER_X: wr = 2 * e; break;
     case ST_UPDATE: case ST_UPDATE_STB: rd += 16 + e; wr = 8 + e + (h->fp32 ? e : 0); break;
-    case ST_UPDATE_X: rd += 16 + e; wr = 8 + 2 * e + (h->fp32 ? e : 0); break;
+    case ST_UPDATE_X: case ST_UPDATE_STB_X: rd += 16 + e; wr = 8 + 2 * e + (h->fp32 ? e : 0); break;
     default: wr = e;
   }
   return (rd + wr) * n;
@@ -650,8 +650,8 @@ const void* psihat_src(const qsm_handle_s* h) {
   return h->fp32 ? (const void*)h->psihat32 : (const void*)h->psihat;
 }
 
-// psi's x- and y-passed fields from psihat (quasi): the first two passes of
-// psi = IDCT(psihat), kept for the transforms of grad psi y and z.  The step
+// psi's x- and y-passed fields from psihat: the first two passes of psi =
+// IDCT(psihat), kept for the transforms of grad psi y and z.  The step
 // forms the x-passed field in its update pass (ST_UPDATE_X), by the default
 // radix plan; the x pass here (set-up, checkpoint read) takes the same plan
 // (variant 0), so either way the field has the same bits.  with_x = false:
@@ -664,10 +664,9 @@ int psi_xy(qsm_handle_s* h, bool with_x = true) {
   return rc;
 }
 
-// psi = IDCT(psihat) into out (quasi.cpp:924-927, 1961); the quasi variant
-// passes through F_PSIX and F_PSIY, which the next step reads.
+// psi = IDCT(psihat) into out (quasi.cpp:924-927, 1961; quasi-stb.cpp:1961)
+// through F_PSIX and F_PSIY, which the next step reads.
 int psi_inverse(qsm_handle_s* h, void* out, bool with_x = true) {
-  if (h->p.variant != QSM_QUASI) return inv3(h, -1, plain_load(psihat_src(h)), out);
   const int rc = psi_xy(h, with_x);
   return rc ? rc : run_pass(h, 2, true, false, plain_load(F(h, F_PSIY)), plain_store(out), false, nullptr);
 }
@@ -987,16 +986,17 @@ int stb_step(qsm_handle_s* h) {
   const void* psih = psihat_src(h);
   const int n0 = h->p.compat_ranks > 0 ? h->p.nx / h->p.compat_ranks : 0;
   if (!h->rec) {  // group kernels (single handles)
-    GroupItem it[4];
+    // sh and grad psi x in one group; grad psi y and z from psi's passed
+    // fields (grad_psi_yz)
+    GroupItem it[3];
     it[0] = {LoadOp{}, F(h, F_F), -1};
     it[0].L.code = LD_SH;
-    for (int c = 0; c < 3; ++c) {
-      it[1 + c] = {LoadOp{}, F(h, F_GX + c), c};
-      it[1 + c].L.code = LD_GRAD_X + c;
-      it[1 + c].L.axis = c;
-    }
-    for (auto& x : it) x.L.a[0] = psih;
-    if ((rc = inv3_group(h, GRP_STB_PSI, it, 4, 1))) return rc;
+    it[1] = {LoadOp{}, F(h, F_GX), 0};
+    it[1].L.code = LD_GRAD_X;
+    it[1].L.axis = 0;
+    for (int o = 0; o < 2; ++o) it[o].L.a[0] = psih;
+    if ((rc = inv3_group(h, GRP_STB_PSI, it, 2, 1))) return rc;
+    if ((rc = grad_psi_yz(h))) return rc;
     L = LoadOp{}; L.code = LD_STRESS_STB3;
     L.a[0] = F(h, F_F); L.a[1] = psi;
     for (int c = 0; c < 3; ++c) L.a[2 + c] = F(h, F_GX + c);
@@ -1020,10 +1020,9 @@ int stb_step(qsm_handle_s* h) {
   } else {
     L.code = LD_SH; L.a[0] = psih;
     if ((rc = inv3(h, -1, L, F(h, F_F)))) return rc;
-    for (int c = 0; c < 3; ++c) {
-      L = LoadOp{}; L.code = LD_GRAD_X + c; L.axis = c; L.a[0] = psih;
-      if ((rc = inv3(h, c, L, F(h, F_GX + c)))) return rc;
-    }
+    L = LoadOp{}; L.code = LD_GRAD_X; L.axis = 0; L.a[0] = psih;
+    if ((rc = inv3(h, 0, L, F(h, F_GX)))) return rc;
+    if ((rc = grad_psi_yz(h))) return rc;
     for (int c = 0; c < 3; ++c) {
       L = LoadOp{}; L.code = LD_STRESS_STB; L.axis = c;
       L.a[0] = F(h, F_F); L.a[1] = psi; L.a[2] = F(h, F_GX + c);
@@ -1038,12 +1037,14 @@ int stb_step(qsm_handle_s* h) {
   L = LoadOp{}; L.code = h->p.hydro == 1 ? LD_NONLIN_STB : LD_NONLIN_STB0;
   L.a[0] = psi; L.a[1] = F(h, F_VX); L.a[2] = F(h, F_VY); L.a[3] = F(h, F_VZ);
   L.a[4] = F(h, F_GX); L.a[5] = F(h, F_GY); L.a[6] = F(h, F_GZ);
-  StoreOp S{}; S.code = ST_UPDATE_STB; S.first = n == 1;
+  // the update pass also runs psi's inverse x pass (into F_PSIX)
+  StoreOp S{}; S.code = ST_UPDATE_STB_X; S.first = n == 1;
   S.d[0] = h->psihat_old; S.d[1] = F(h, F_NHAT_OLD); S.d[2] = h->psihat32; S.s[0] = h->psihat;
+  S.d[3] = F(h, F_PSIX);
   if (mixed) { S.dc = h->dsc + D_NSUM; S.dc_old = h->dsc + D_NHAT0; S.dc_out = h->dsc + D_NHAT0; }
   if ((rc = fwd3(h, -1, L, F(h, F_NTMP), S, mixed ? D_NSUM : -1))) return rc;
   std::swap(h->psihat, h->psihat_old);
-  if ((rc = inv3(h, -1, plain_load(psihat_src(h)), F(h, h->psi_old)))) return rc;
+  if ((rc = psi_inverse(h, F(h, h->psi_old), false))) return rc;
   std::swap(h->psi_cur, h->psi_old);
   return QSM_OK;
 }
@@ -2053,11 +2054,12 @@ extern "C" int qsm_checkpoint_read(qsm_handle h, const char* path) {
   fclose(fp);
   if (!ok) return fail(QSM_ERR_ARG, "qsm_checkpoint_read: %s is truncated", path);
   if (h->fp32) HIPCHK(launch_round32(h->psihat, h->psihat32, (long)h->n, h->stream));
-  if (h->p.variant == QSM_QUASI) {
-    // psi's and rho's passed fields, by the passes the step runs
+  {
+    // psi's (and in the quasi variant rho's) passed fields, by the passes
+    // the step runs
     int rc = psi_xy(h);
     h->rhoy = 0;
-    if (!rc) rc = rho_inverse(h, nullptr);
+    if (!rc && h->p.variant == QSM_QUASI) rc = rho_inverse(h, nullptr);
     if (rc) return rc;
   }
   h->step = hd.step;

@@ -69,7 +69,7 @@ long pass_blocks(int prec_fp32, int n, int axis, const Geo& g);
 enum GroupCode : int {
   GRP_PSI = 0,     // F, G, grad psi x, lap grad psi (x, y, z) from psihat
   GRP_PROJ,        // p, v_x, v_y, v_z from S-hat (x, y, z) and D-hat
-  GRP_STB_PSI,     // sh, grad psi (x, y, z)
+  GRP_STB_PSI,     // sh, grad psi x
   GRP_VEL,         // v_x, v_y, v_z (stb, and quasi steps whose p is read by nothing)
   GRP_COUNT
 };

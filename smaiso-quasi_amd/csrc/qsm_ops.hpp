@@ -80,16 +80,20 @@ enum StoreCode : int {
   // The same stores, then the inverse x transform (DCT-III) of the stored
   // spectral values into d3: the field's first inverse pass, run on the tile
   // the forward x pass holds (rho's x-passed field after the density filter;
-  // psi's after the update, from psihat_new, its FP32 copy in mixed precision)
+  // psi's after either update, from psihat_new, its FP32 copy in mixed
+  // precision)
   ST_FILTER_X,
   ST_UPDATE_X,
+  ST_UPDATE_STB_X,
 };
 
 // the store operator an X code extends, and whether a code has the x inverse
 __host__ __device__ constexpr int st_base(int st) {
-  return st == ST_FILTER_X ? ST_FILTER : (st == ST_UPDATE_X ? ST_UPDATE : st);
+  return st == ST_FILTER_X ? ST_FILTER
+         : st == ST_UPDATE_X ? ST_UPDATE
+         : st == ST_UPDATE_STB_X ? ST_UPDATE_STB : st;
 }
-__host__ __device__ constexpr bool st_inverts(int st) { return st == ST_FILTER_X || st == ST_UPDATE_X; }
+__host__ __device__ constexpr bool st_inverts(int st) { return st_base(st) != st; }
 
 struct LoadOp {
   int code;

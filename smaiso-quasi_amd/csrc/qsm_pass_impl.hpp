@@ -1275,9 +1275,10 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK)) k_pass(const Pa
 }
 
 // Forward x passes that also run the x inverse of their stored values
-// (ST_FILTER_X, ST_UPDATE_X): the register allocator is held to the two
-// workgroups per CU that the LDS allows (left alone it takes 132 VGPRs in
-// FP32 at 512, one workgroup per CU).
+// (ST_FILTER_X, ST_UPDATE_X, ST_UPDATE_STB_X): the register allocator is held
+// to two workgroups per CU.  Left alone it takes 132 VGPRs in FP32 at 512, one
+// 512-thread workgroup per CU instead of the two the LDS allows.  Smaller
+// workgroups are left alone (held to 128 VGPRs, the 96-point tile spills).
 template <typename T, int N, int V>
 constexpr int inv_store_waves() {
   constexpr int w = (TileCfg<T, N, kAxX, V>::BLOCK + 255) / 256;  // waves per SIMD per workgroup
@@ -1389,7 +1390,8 @@ struct GroupOps<GRP_PROJ> {
 };
 template <>
 struct GroupOps<GRP_STB_PSI> {
-  using L = OpList<LD_SH, LD_GRAD_X, LD_GRAD_Y, LD_GRAD_Z>;
+  // grad psi y and z start from psi's passed fields, as in the quasi step
+  using L = OpList<LD_SH, LD_GRAD_X>;
   static constexpr bool INV = true;
   static constexpr int AX = kAxX;
   static constexpr bool XORD = false;
@@ -1613,6 +1615,22 @@ __global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
                                       b - prefix[lo], sm);
 }
 
+// the batched form of k_pass_sx (the same register bound)
+template <typename T, int N, int AX, bool INV, int LD, int ST, int V>
+__global__ void __launch_bounds__((TileCfg<T, N, AX, V>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(inv_store_waves<T, N, V>()))) k_pass_batch_sx(const PassArgs* __restrict__ descs, const int* __restrict__ prefix, int count) {
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int lo = 0, hi = count;  // invariant: prefix[lo] <= b < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid;
+  }
+  __shared__ SmemFor<T, N, AX, INV, LD, ST, V> sm;
+  const PassArgs& A = descs[lo];
+  pass_tile<T, N, AX, INV, LD, ST, V>(A.geo, A.waves, A.phys, A.ld, A.st, A.tw, A.part, A.dst,
+                                      b - prefix[lo], sm);
+}
+
 template <typename T, int N, int AX, int V = 0>
 long tiles_of(const Geo& g) {
   using C = TileCfg<T, N, AX, V>;
@@ -1654,8 +1672,12 @@ hipError_t launch_one(const PassArgs& A, long* nblocks_out, hipStream_t s, const
   using C = TileCfg<T, N, AX, V>;
   if (B) {
     if (B->total_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_pass_batch<T, N, AX, INV, LD, ST, V>), dim3((unsigned)B->total_tiles),
-                       dim3(C::BLOCK), 0, s, B->descs, B->prefix, B->count);
+    if constexpr (st_inverts(ST))
+      hipLaunchKernelGGL((k_pass_batch_sx<T, N, AX, INV, LD, ST, V>), dim3((unsigned)B->total_tiles),
+                         dim3(C::BLOCK), 0, s, B->descs, B->prefix, B->count);
+    else
+      hipLaunchKernelGGL((k_pass_batch<T, N, AX, INV, LD, ST, V>), dim3((unsigned)B->total_tiles),
+                         dim3(C::BLOCK), 0, s, B->descs, B->prefix, B->count);
     return hipGetLastError();
   }
   const long nb = tiles_of<T, N, AX, V>(A.geo);
@@ -1757,6 +1779,7 @@ hipError_t dispatch_ops(bool inv, int ld, int st, int variant, const PassArgs& A
       switch (st) {
 #define QSM_S(OP) case OP: return launch_one<T, N, AX, false, LD_PLAIN, OP>(A, nb, s, B);
         QSM_S(ST_FILTER) QSM_S(ST_UPDATE) QSM_S(ST_UPDATE_STB) QSM_S(ST_FILTER_X) QSM_S(ST_UPDATE_X)
+        QSM_S(ST_UPDATE_STB_X)
 #undef QSM_S
         default: break;
       }

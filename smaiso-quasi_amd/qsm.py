@@ -32,7 +32,7 @@ LOAD_OPS = ["plain", "zero", "coef_f", "coef_g", "grad_x", "grad_y", "grad_z", "
             "stress", "pressure", "vel0", "vel1", "vel2", "scale", "nonlin", "nonlin_p",
             "stress_stb", "nonlin_stb", "nonlin_stb0", "stress3", "stress_stb3", "stress3_divv",
             "nonlin_py"]
-STORE_OPS = ["plain", "filter", "update", "update_stb", "filter_x", "update_x"]
+STORE_OPS = ["plain", "filter", "update", "update_stb", "filter_x", "update_x", "update_stb_x"]
 
 FIELDS = [
     "PSI", "RHO", "VX", "VY", "VZ", "P", "DIVV", "PSIHAT", "F", "G", "GX", "GY", "GZ",
