@@ -33,7 +33,9 @@ def symbol(k, fp32):
     t = "float" if fp32 else "double"
     if k["ld"] == 63:
         return "qsm::k_finish_batch (FP64 mean-mode sums)"
-    return (f"void qsm::k_pass_batch<{t}, {k['n']}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
+    # forward x passes that also run the x inverse of what they store
+    kern = "k_pass_batch_sx" if qsm.STORE_OPS[k["st"]].endswith("_x") else "k_pass_batch"
+    return (f"void qsm::{kern}<{t}, {k['n']}, {k['axis']}, {'true' if k['inv'] else 'false'}, "
             f"{k['ld']}, {k['st']}, {k['variant']}>")
 
 
