@@ -247,9 +247,10 @@ int qsm_timing_enable(qsm_handle h, int enable);
 int qsm_timing_get(qsm_handle h, int which, double* total_ms, long long* launches,
                    double* alg_bytes);
 /* the rank-th kernel (0 = most device time) among those timed: the pass
- * kernel k_pass<T, N, axis, inv, ld, st, variant> with its load and store
- * operator codes and tile variant, its total time, launches and algorithmic
- * bytes */
+ * kernel k_pass<T, N, axis, inv, ld, st, variant> (k_pass_sx for the store
+ * operators that also run the x inverse of what they store) with its load
+ * and store operator codes and tile variant, its total time, launches and
+ * algorithmic bytes */
 int qsm_timing_kernel(qsm_handle h, int rank, int* axis, int* inv, int* ld, int* st, int* variant,
                       double* total_ms, long long* launches, double* alg_bytes);
 /* reset the counters and time only the launches of one kernel, named by the
